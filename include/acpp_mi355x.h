@@ -1,0 +1,226 @@
+/*
+ * acpp_mi355x.h -- C-ABI of the MI355X-native device-wide primitives.
+ *
+ * This is the drop-in boundary for AdaptiveCpp's data-parallel hot path:
+ * the acpp::algorithms reduce / transform_reduce / inclusive_scan /
+ * exclusive_scan entry points, sycl::reduction, and the atomic fetch
+ * operations that stdpar offloading lowers onto.  The C++ facade in
+ * include/acpp/algorithms/numeric.hpp re-declares the reference's templates
+ * on top of these functions; ctypes / cgo / JNI bindings call them directly.
+ *
+ * Conventions shared by every entry point:
+ *   - Pointers to data are device pointers (hipMalloc / hipMallocManaged
+ *     memory); `n` is an element count.
+ *   - `stream` is a hipStream_t passed as an opaque pointer (NULL = the null
+ *     stream).  Every compute entry point is asynchronous with respect to the
+ *     host: it enqueues work on `stream` and returns, like the reference,
+ *     whose calls return a sycl::event (doc/algorithms.md:35-37).
+ *   - `scratch` is an allocation group (allocation_cache.hpp:142-181).  The
+ *     memory it hands out stays owned by the group until
+ *     acpp_allocation_group_release / _destroy; the caller must keep the
+ *     group alive until the enqueued work has finished, or order all users
+ *     of one cache on one stream (allocation_cache.hpp:129-141).
+ *   - Nothing throws.  Every function returns an acpp_status; on failure
+ *     acpp_last_error_message() describes the problem (thread-local).
+ *   - n == 0: reduce, transform_reduce and the scans enqueue nothing and
+ *     leave `*out` untouched (numeric.hpp:165-169, decoupled_lookback_scan.hpp:565-566).
+ *   - `config` may be NULL; zero fields mean "library's choice".
+ */
+#ifndef ACPP_MI355X_H
+#define ACPP_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+typedef int acpp_status;
+#define ACPP_SUCCESS 0
+#define ACPP_ERROR_INVALID_ARGUMENT 1 /* null pointer, bad enum, size limit */
+#define ACPP_ERROR_UNSUPPORTED 2      /* op not defined for this element type */
+#define ACPP_ERROR_OUT_OF_MEMORY 3
+#define ACPP_ERROR_HIP 4              /* HIP runtime reported an error */
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *acpp_last_error_message(void);
+
+/* Library version as "major.minor.patch". */
+const char *acpp_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Operators.  Values equal __acpp_sscp_algorithm_op                        */
+/* (include/hipSYCL/sycl/libkernel/sscp/builtins/builtin_config.hpp:27-37). */
+/* Integer types accept all nine; floating-point types accept plus,         */
+/* multiply, min and max.                                                   */
+/* ------------------------------------------------------------------------ */
+typedef int32_t acpp_algorithm_op;
+#define ACPP_OP_PLUS 0
+#define ACPP_OP_MULTIPLY 1
+#define ACPP_OP_MIN 2
+#define ACPP_OP_MAX 3
+#define ACPP_OP_BIT_AND 4
+#define ACPP_OP_BIT_OR 5
+#define ACPP_OP_BIT_XOR 6
+#define ACPP_OP_LOGICAL_AND 7
+#define ACPP_OP_LOGICAL_OR 8
+
+typedef void *acpp_stream_t; /* hipStream_t */
+
+/* ------------------------------------------------------------------------ */
+/* Launch configuration.  0 in a field = library's choice (autotuned table  */
+/* or learned cost model).  Replaces the reference's hard-coded local size  */
+/* 128 and groups = 4 x compute units (numeric.hpp:144-156,                  */
+/* memory_streaming.hpp:33-46) and its 16 items per work-item scan tiles    */
+/* (decoupled_lookback_scan.hpp:481-488, scan.hpp:26-31).                   */
+/* ------------------------------------------------------------------------ */
+typedef struct acpp_launch_config {
+  uint32_t work_group_size;  /* 64..1024, multiple of 64 */
+  uint32_t items_per_thread; /* reduce: 16-byte loads in flight per thread
+                                (2, 4 or 8); scan: 16-byte rows per thread
+                                (2, 4 or 8) */
+  uint32_t groups_per_cu;    /* reduce only: resident work-groups per CU */
+  uint32_t flags;            /* reserved, 0 */
+} acpp_launch_config;
+
+/* ------------------------------------------------------------------------ */
+/* Scratch memory: allocation_cache / allocation_group                      */
+/* (include/hipSYCL/algorithms/util/allocation_cache.hpp:42-181).           */
+/* ------------------------------------------------------------------------ */
+typedef struct acpp_allocation_cache acpp_allocation_cache;
+typedef struct acpp_allocation_group acpp_allocation_group;
+
+#define ACPP_ALLOCATION_DEVICE 0 /* hipMalloc */
+#define ACPP_ALLOCATION_SHARED 1 /* hipMallocManaged */
+#define ACPP_ALLOCATION_HOST 2   /* hipHostMalloc */
+
+/* allocation_cache(allocation_type) -- allocation_cache.hpp:44-45 */
+acpp_status acpp_allocation_cache_create(int allocation_type,
+                                         acpp_allocation_cache **out);
+/* allocation_cache::purge -- allocation_cache.hpp:51-62.  Frees every
+   allocation currently returned to the cache. */
+acpp_status acpp_allocation_cache_purge(acpp_allocation_cache *cache);
+/* ~allocation_cache -- allocation_cache.hpp:47-49 */
+acpp_status acpp_allocation_cache_destroy(acpp_allocation_cache *cache);
+/* allocation_group(cache, device) -- allocation_cache.hpp:144-146 */
+acpp_status acpp_allocation_group_create(acpp_allocation_cache *cache,
+                                         int device,
+                                         acpp_allocation_group **out);
+/* allocation_group::obtain<T>(count) -- allocation_cache.hpp:169-175 */
+acpp_status acpp_allocation_group_obtain(acpp_allocation_group *group,
+                                         size_t bytes, size_t alignment,
+                                         void **out);
+/* allocation_group::release -- allocation_cache.hpp:162-167 */
+acpp_status acpp_allocation_group_release(acpp_allocation_group *group);
+/* ~allocation_group -- allocation_cache.hpp:158-160 */
+acpp_status acpp_allocation_group_destroy(acpp_allocation_group *group);
+
+/* Number of compute units of `device` (the value the reference reads as
+   info::device::max_compute_units, numeric.hpp:154-156). */
+acpp_status acpp_device_compute_units(int device, int *out);
+
+/* ------------------------------------------------------------------------ */
+/* reduce: *out = op(init, x[0] op x[1] op ... op x[n-1])                    */
+/* Replaces algorithms::reduce (numeric.hpp:226-249), i.e.                   */
+/* transform_reduce with an identity transform (numeric.hpp:188-202).       */
+/* One launch: grid-stride 128-bit loads, DPP wave reduction, LDS combine,  */
+/* and the last work-group to finish folds the per-group partials.          */
+/* ------------------------------------------------------------------------ */
+acpp_status acpp_reduce_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *out, int8_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *out, int16_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *out, int32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *out, int64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *out, uint8_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *out, uint16_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *out, uint32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *out, uint64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, float init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_reduce_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *out, double init, acpp_algorithm_op op, const acpp_launch_config *config);
+/* bf16 input (raw uint16 bit patterns), fp32 accumulation and result.  No
+   counterpart in the reference (it has no bf16 type); same contract. */
+acpp_status acpp_reduce_bf16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, float *out, float init, acpp_algorithm_op op, const acpp_launch_config *config);
+
+/* Compensated sums (op is always plus): fp32 accumulation with TwoSum error
+   terms carried per lane and per wave, FP64 only in the final cross-group
+   combine.  Deterministic: fixed reduction tree, no floating-point atomics.
+   Precision extension of algorithms::reduce (numeric.hpp:226-249). */
+acpp_status acpp_reduce_compensated_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, float init, const acpp_launch_config *config);
+acpp_status acpp_reduce_compensated_bf16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, float *out, float init, const acpp_launch_config *config);
+
+/* ------------------------------------------------------------------------ */
+/* transform_reduce, binary form with the default operators                 */
+/* (numeric.hpp:204-223): *out = init + sum_i first1[i] * first2[i].        */
+/* ------------------------------------------------------------------------ */
+acpp_status acpp_transform_reduce_dot_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first1, uint64_t n, const int32_t *first2, int32_t *out, int32_t init, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_dot_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first1, uint64_t n, const int64_t *first2, int64_t *out, int64_t init, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_dot_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first1, uint64_t n, const uint32_t *first2, uint32_t *out, uint32_t init, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_dot_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first1, uint64_t n, const uint64_t *first2, uint64_t *out, uint64_t init, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_dot_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first1, uint64_t n, const float *first2, float *out, float init, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_dot_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first1, uint64_t n, const double *first2, double *out, double init, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_dot_bf16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first1, uint64_t n, const uint16_t *first2, float *out, float init, const acpp_launch_config *config);
+/* Compensated dot products (Dot2: TwoProduct + TwoSum per element, FP64 in
+   the final cross-group combine). */
+acpp_status acpp_transform_reduce_dot_compensated_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first1, uint64_t n, const float *first2, float *out, float init, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_dot_compensated_bf16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first1, uint64_t n, const uint16_t *first2, float *out, float init, const acpp_launch_config *config);
+
+/* ------------------------------------------------------------------------ */
+/* sycl::reduction over a 1-D range (handler.hpp:799-975 with              */
+/* reduction.hpp:165-236 and reduction_descriptor.hpp:25-54):                */
+/*   initialize_to_identity != 0 : *out = result                            */
+/*   initialize_to_identity == 0 : *out = op(*out, result)                  */
+/* where result is the reduction of x[0..n) (the identity when n == 0).     */
+/* ------------------------------------------------------------------------ */
+acpp_status acpp_reduction_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+
+/* ------------------------------------------------------------------------ */
+/* Scans: single-pass decoupled look-back (decoupled_lookback_scan.hpp:     */
+/* 555-655), one launch, no status-initialisation kernel.                   */
+/*   inclusive_scan (numeric.hpp:253-279):                                  */
+/*     d[i] = [init op] x[0] op ... op x[i];  init == NULL means no init.   */
+/*   exclusive_scan (numeric.hpp:281-297):                                  */
+/*     d[0] = init, d[i] = init op x[0] op ... op x[i-1].                   */
+/* d_first may equal first (in-place); other overlaps are undefined.        */
+/* Integer results are exact; floating-point results depend on the order   */
+/* in which the look-back finds predecessors and may differ in the last    */
+/* bits between runs.                                                       */
+/* ------------------------------------------------------------------------ */
+acpp_status acpp_inclusive_scan_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *d_first, acpp_algorithm_op op, const int8_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *d_first, acpp_algorithm_op op, const int16_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, acpp_algorithm_op op, const int32_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, acpp_algorithm_op op, const int64_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *d_first, acpp_algorithm_op op, const uint8_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *d_first, acpp_algorithm_op op, const uint16_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *d_first, acpp_algorithm_op op, const uint32_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *d_first, acpp_algorithm_op op, const uint64_t *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, acpp_algorithm_op op, const float *init, const acpp_launch_config *config);
+acpp_status acpp_inclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, acpp_algorithm_op op, const double *init, const acpp_launch_config *config);
+
+acpp_status acpp_exclusive_scan_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *d_first, int8_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *d_first, int16_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, int32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, int64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *d_first, uint8_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *d_first, uint16_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *d_first, uint32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *d_first, uint64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, float init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_exclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, double init, acpp_algorithm_op op, const acpp_launch_config *config);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* ACPP_MI355X_H */

@@ -1,0 +1,13 @@
+"""MI355X-native device-wide primitives with AdaptiveCpp's acpp::algorithms API.
+
+The compute path is the HIP library ``lib/libacpp_mi355x.so`` (gfx950),
+reached through the C-ABI in ``include/acpp_mi355x.h``.
+"""
+from . import _abi  # noqa: F401
+from .algorithms import (ALLOCATION_DEVICE, ALLOCATION_HOST, ALLOCATION_SHARED,  # noqa: F401
+                         BIT_AND, BIT_OR, BIT_XOR, LOGICAL_AND, LOGICAL_OR, MAX, MIN, MULTIPLY,
+                         PLUS, AllocationCache, AllocationGroup, Queue, compute_units,
+                         exclusive_scan, inclusive_scan, reduce, reduce_compensated, reduction,
+                         transform_reduce)
+
+__version__ = "0.1.0"

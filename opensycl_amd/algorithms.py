@@ -1,0 +1,268 @@
+"""Python mirror of ``acpp::algorithms`` (include/hipSYCL/algorithms/numeric.hpp).
+
+Every function has the reference's argument order -- queue, scratch
+allocation group, input range, output, init, operator -- with torch device
+tensors standing for the iterator ranges.  Calls are asynchronous on the
+queue's stream, like the reference's (doc/algorithms.md:35-37); use
+``queue.wait()`` before reading results on the host.
+
+The device work is done by the HIP library through its C-ABI
+(``opensycl_amd._abi``); nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+from . import _abi
+from ._abi import LaunchConfig, check
+
+# Operator codes (== __acpp_sscp_algorithm_op, builtin_config.hpp:27-37)
+PLUS = 0
+MULTIPLY = 1
+MIN = 2
+MAX = 3
+BIT_AND = 4
+BIT_OR = 5
+BIT_XOR = 6
+LOGICAL_AND = 7
+LOGICAL_OR = 8
+
+_OP_NAMES = {
+    "plus": PLUS, "multiplies": MULTIPLY, "multiply": MULTIPLY, "minimum": MIN, "min": MIN,
+    "maximum": MAX, "max": MAX, "bit_and": BIT_AND, "bit_or": BIT_OR, "bit_xor": BIT_XOR,
+    "logical_and": LOGICAL_AND, "logical_or": LOGICAL_OR,
+}
+
+ALLOCATION_DEVICE = 0
+ALLOCATION_SHARED = 1
+ALLOCATION_HOST = 2
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _tag(dtype) -> str:
+    torch = _torch()
+    table = {
+        torch.int8: "i8", torch.int16: "i16", torch.int32: "i32", torch.int64: "i64",
+        torch.uint8: "u8", torch.uint16: "u16", torch.uint32: "u32", torch.uint64: "u64",
+        torch.float32: "f32", torch.float64: "f64", torch.bfloat16: "bf16",
+    }
+    if dtype not in table:
+        raise TypeError(f"unsupported element type {dtype}")
+    return table[dtype]
+
+
+def _op(op) -> int:
+    if isinstance(op, str):
+        if op not in _OP_NAMES:
+            raise ValueError(f"unknown operator {op!r}")
+        return _OP_NAMES[op]
+    return int(op)
+
+
+def _cfg(config):
+    if config is None:
+        return None
+    if isinstance(config, LaunchConfig):
+        return ctypes.byref(config)
+    if isinstance(config, dict):
+        return ctypes.byref(LaunchConfig(**config))
+    wg, ipt, gpc = (tuple(config) + (0, 0, 0))[:3]
+    return ctypes.byref(LaunchConfig(wg, ipt, gpc, 0))
+
+
+class Queue:
+    """In-order queue on a HIP stream (torch stream or raw hipStream_t)."""
+
+    def __init__(self, stream=None, device: int = 0):
+        torch = _torch()
+        self.device = device
+        if stream is None:
+            stream = torch.cuda.current_stream(device)
+        self.stream = stream
+
+    @property
+    def handle(self) -> int:
+        s = self.stream
+        if isinstance(s, int):
+            return s
+        return s.cuda_stream
+
+    def wait(self) -> None:
+        if isinstance(self.stream, int):
+            _torch().cuda.synchronize(self.device)
+        else:
+            self.stream.synchronize()
+
+
+class AllocationCache:
+    """util::allocation_cache (allocation_cache.hpp:42-127)."""
+
+    def __init__(self, allocation_type: int = ALLOCATION_DEVICE):
+        lib = _abi.load()
+        h = ctypes.c_void_p()
+        check(lib.acpp_allocation_cache_create(allocation_type, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def purge(self) -> None:
+        check(_abi.load().acpp_allocation_cache_purge(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            check(_abi.load().acpp_allocation_cache_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class AllocationGroup:
+    """util::allocation_group (allocation_cache.hpp:142-181)."""
+
+    def __init__(self, cache: AllocationCache, device: int = 0):
+        lib = _abi.load()
+        h = ctypes.c_void_p()
+        check(lib.acpp_allocation_group_create(cache.handle, device, ctypes.byref(h)))
+        self._h = h
+        self._cache = cache
+
+    @property
+    def handle(self):
+        return self._h
+
+    def obtain(self, nbytes: int, alignment: int = 256) -> int:
+        p = ctypes.c_void_p()
+        check(_abi.load().acpp_allocation_group_obtain(self._h, nbytes, alignment,
+                                                       ctypes.byref(p)))
+        return p.value
+
+    def release(self) -> None:
+        check(_abi.load().acpp_allocation_group_release(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            check(_abi.load().acpp_allocation_group_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def compute_units(device: int = 0) -> int:
+    v = ctypes.c_int()
+    check(_abi.load().acpp_device_compute_units(device, ctypes.byref(v)))
+    return v.value
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return 0
+    if not t.is_cuda:
+        raise ValueError("tensors must live in device memory")
+    return t.data_ptr()
+
+
+def _check_contig(*ts):
+    for t in ts:
+        if t is not None and not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+
+def _scalar(tag: str, v):
+    ct = _abi.SCALAR_TAGS.get(tag, ctypes.c_float)
+    return ct(v)
+
+
+def reduce(q: Queue, scratch: AllocationGroup, x, out, init=0, op=PLUS, config=None) -> None:
+    """algorithms::reduce (numeric.hpp:226-249): out[0] = op(init, fold(x))."""
+    _check_contig(x, out)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_reduce_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), _scalar(tag, init),
+             _op(op), _cfg(config)))
+
+
+def reduce_compensated(q: Queue, scratch: AllocationGroup, x, out, init=0.0,
+                       config=None) -> None:
+    """Compensated fp32 sum of fp32 or bf16 input into an fp32 out[0]."""
+    _check_contig(x, out)
+    tag = _tag(x.dtype)
+    if tag not in ("f32", "bf16"):
+        raise TypeError("compensated sums take float32 or bfloat16 input")
+    fn = getattr(_abi.load(), f"acpp_reduce_compensated_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), ctypes.c_float(init),
+             _cfg(config)))
+
+
+def transform_reduce(q: Queue, scratch: AllocationGroup, x1, x2, out, init=0,
+                     compensated: bool = False, config=None) -> None:
+    """Binary transform_reduce with plus / multiplies (numeric.hpp:204-223)."""
+    _check_contig(x1, x2, out)
+    if x1.numel() > x2.numel() or x1.dtype != x2.dtype:
+        raise ValueError("second range must have the first range's type and length")
+    tag = _tag(x1.dtype)
+    lib = _abi.load()
+    if compensated:
+        if tag not in ("f32", "bf16"):
+            raise TypeError("compensated dot products take float32 or bfloat16 input")
+        fn = getattr(lib, f"acpp_transform_reduce_dot_compensated_{tag}")
+        init_c = ctypes.c_float(init)
+    else:
+        fn = getattr(lib, f"acpp_transform_reduce_dot_{tag}")
+        init_c = ctypes.c_float(init) if tag == "bf16" else _scalar(tag, init)
+    check(fn(scratch.handle, q.handle, _ptr(x1), x1.numel(), _ptr(x2), _ptr(out), init_c,
+             _cfg(config)))
+
+
+def reduction(q: Queue, scratch: AllocationGroup, x, out, op=PLUS,
+              initialize_to_identity: bool = False, config=None) -> None:
+    """parallel_for(range{n}, sycl::reduction(out, op[, initialize_to_identity]))."""
+    _check_contig(x, out)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_reduction_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), _op(op),
+             1 if initialize_to_identity else 0, _cfg(config)))
+
+
+def inclusive_scan(q: Queue, scratch: AllocationGroup, x, d, op=PLUS, init=None,
+                   config=None) -> None:
+    """algorithms::inclusive_scan (numeric.hpp:253-279)."""
+    _check_contig(x, d)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_inclusive_scan_{tag}")
+    init_p = None
+    if init is not None:
+        init_c = _scalar(tag, init)
+        init_p = ctypes.addressof(init_c)
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _op(op), init_p,
+             _cfg(config)))
+
+
+def exclusive_scan(q: Queue, scratch: AllocationGroup, x, d, init=0, op=PLUS,
+                   config=None) -> None:
+    """algorithms::exclusive_scan (numeric.hpp:281-297)."""
+    _check_contig(x, d)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_exclusive_scan_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _scalar(tag, init),
+             _op(op), _cfg(config)))

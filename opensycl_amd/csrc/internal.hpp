@@ -1,0 +1,103 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <type_traits>
+
+#include "acpp_mi355x.h"
+#include "acpp/device/ops.hpp"
+#include "acpp/device/wave.hpp"
+
+namespace acpp_impl {
+
+namespace ops = ::acpp::ops;
+namespace wave = ::acpp::wave;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+acpp_status set_error(acpp_status code, const std::string &msg);
+acpp_status clear_error();
+acpp_status hip_error(hipError_t e, const char *what);
+
+#define ACPP_CHECK_HIP(expr)                                                   \
+  do {                                                                         \
+    hipError_t acpp_e_ = (expr);                                               \
+    if (acpp_e_ != hipSuccess) return ::acpp_impl::hip_error(acpp_e_, #expr);  \
+  } while (0)
+
+// Check the launch that was just enqueued.
+acpp_status check_launch(const char *kernel_name);
+
+// ---------------------------------------------------------------------------
+// Synchronisation regions.
+//
+// A sync region is scratch memory that is zero when first handed out and
+// that every kernel using it leaves in a reusable state:
+//   - counters (tile ticket, done counter, group ticket) are reset to zero by
+//     the last work-group of the launch;
+//   - scan status words carry the launch's epoch, so words written by an
+//     earlier launch are recognised as stale without clearing them.
+// This removes the reference's separate status-initialisation kernel
+// (decoupled_lookback_scan.hpp:584-589) and any memset from the hot path.
+// ---------------------------------------------------------------------------
+struct sync_header {
+  uint32_t epoch;         // last epoch used by a completed launch (0 = none)
+  uint32_t tile_counter;  // dynamic tile ticket
+  uint32_t done_counter;  // number of work-groups that finished
+  uint32_t pad[61];       // header occupies 256 bytes
+};
+static_assert(sizeof(sync_header) == 256, "sync header is 256 bytes");
+
+// Epochs live in 30 bits of a status word; 0 is never a live epoch.
+constexpr uint32_t max_epoch = (1u << 30) - 1;
+
+}  // namespace acpp_impl
+
+// Internal part of the allocation group used by the kernels.
+struct acpp_allocation_group;
+namespace acpp_impl {
+// Obtain ordinary scratch from the group.
+acpp_status group_obtain(acpp_allocation_group *g, size_t bytes, void **out);
+// Obtain a sync region of at least `bytes` (header included).  Fresh regions
+// are zeroed on `stream` before first use.  `capacity_bytes` receives the
+// region's real size.
+acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
+                              hipStream_t stream, void **out,
+                              size_t *capacity_bytes);
+int group_device(const acpp_allocation_group *g);
+// Make the group's device current; returns its compute-unit count.
+acpp_status enter_device(acpp_allocation_group *g, int *compute_units);
+
+// ---------------------------------------------------------------------------
+// Element types
+// ---------------------------------------------------------------------------
+struct bf16_t {
+  uint16_t bits;
+};
+
+template <class T>
+struct elem_traits {
+  using storage = T;
+  // Type the kernels compute in.
+  using work = std::conditional_t<
+      (std::is_integral_v<T> && sizeof(T) < 4),
+      std::conditional_t<std::is_signed_v<T>, int32_t, uint32_t>, T>;
+  __host__ __device__ static work to_work(T x) { return static_cast<work>(x); }
+  __host__ __device__ static T from_work(work x) { return static_cast<T>(x); }
+};
+
+template <>
+struct elem_traits<bf16_t> {
+  using storage = uint16_t;
+  using work = float;
+  __host__ __device__ static float to_work(bf16_t x) {
+    return __builtin_bit_cast(float, static_cast<uint32_t>(x.bits) << 16);
+  }
+};
+
+}  // namespace acpp_impl

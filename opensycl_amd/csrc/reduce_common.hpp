@@ -1,0 +1,271 @@
+// Device-wide reduction kernel shared by reduce, transform_reduce (dot),
+// sycl::reduction and the compensated sums.
+//
+// Reference behaviour being replaced (include/hipSYCL/algorithms/):
+//   numeric.hpp:67-135 wg_model_reduction: local size 128, groups =
+//     4 x compute units, a grid-stride loop of scalar loads per work-item
+//     (util/memory_streaming.hpp:86-91), an LDS tree per group
+//     (reduction/wg_model/group_reduction_algorithms.hpp:139-152), then
+//     ceil(log_128(groups)) further launches over the group partials
+//     (reduction/reduction_engine.hpp:123-142, 211-232) and a final
+//     *out = op(init, r) (reduction/reduction_descriptor.hpp:36-38).
+//
+// MI355X design (one launch):
+//   - each work-item keeps UNROLL independent accumulators and issues UNROLL
+//     128-bit loads per iteration of a grid-stride loop over 16-byte chunks;
+//     misaligned head/tail elements go through a scalar grid-stride loop;
+//   - the work-group reduces with DPP inside each wave and one LDS exchange
+//     across waves;
+//   - every group stores its partial, then takes a ticket; the group that
+//     draws the last ticket folds all partials in a fixed order, writes
+//     *out and resets the ticket to zero for the next launch.
+//   Floating-point results are deterministic for a given launch geometry.
+#pragma once
+
+#include "internal.hpp"
+#include "acpp/device/wave.hpp"
+
+namespace acpp_impl {
+namespace reduce_detail {
+
+constexpr int max_waves = 16;  // work-groups of up to 1024 work-items
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Monoids: how a work-item accumulates elements and how partials combine.
+// ---------------------------------------------------------------------------
+template <class W, class Op>
+struct plain_monoid {
+  using value = W;
+  static constexpr bool compensated = false;
+  __device__ __forceinline__ static W identity() { return Op::identity(); }
+  __device__ __forceinline__ static W apply(W a, W b) { return Op::apply(a, b); }
+  __device__ __forceinline__ static W insert(W acc, W x) { return Op::apply(acc, x); }
+  // Dot products: plus of products (numeric.hpp:215-223 defaults).
+  __device__ __forceinline__ static W insert2(W acc, W x, W y) {
+    return ops::wrap_add(acc, ops::wrap_mul(x, y));
+  }
+};
+
+// fp32 sum with a running TwoSum error term.
+struct comp_value {
+  float s;
+  float c;
+};
+
+struct comp_monoid {
+  using value = comp_value;
+  static constexpr bool compensated = true;
+  __device__ __forceinline__ static comp_value identity() { return {-0.0f, 0.0f}; }
+
+  // Knuth's TwoSum: s + err == a + b exactly.
+  __device__ __forceinline__ static float two_sum(float a, float b, float &err) {
+#pragma clang fp contract(off)
+    float s = a + b;
+    float bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+    return s;
+  }
+  __device__ __forceinline__ static comp_value apply(comp_value a, comp_value b) {
+#pragma clang fp contract(off)
+    float e;
+    float s = two_sum(a.s, b.s, e);
+    return {s, (a.c + b.c) + e};
+  }
+  __device__ __forceinline__ static comp_value insert(comp_value acc, float x) {
+#pragma clang fp contract(off)
+    float e;
+    float s = two_sum(acc.s, x, e);
+    return {s, acc.c + e};
+  }
+  // Dot2 (Ogita, Rump, Oishi 2005): TwoProduct via FMA, then TwoSum.
+  __device__ __forceinline__ static comp_value insert2(comp_value acc, float x, float y) {
+#pragma clang fp contract(off)
+    float p = x * y;
+    float ep = __builtin_fmaf(x, y, -p);
+    float e;
+    float s = two_sum(acc.s, p, e);
+    return {s, acc.c + (e + ep)};
+  }
+};
+
+template <class W>
+struct double_plus {
+  using value = double;
+  __device__ __forceinline__ static double identity() { return -0.0; }
+  __device__ __forceinline__ static double apply(double a, double b) { return a + b; }
+};
+
+// Reduce a value over the work-group; the result is valid in wave 0.
+template <class M>
+__device__ __forceinline__ typename M::value block_reduce(typename M::value v,
+                                                          typename M::value *s_waves) {
+  const int nw = blockDim.x >> 6;
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  v = wave::reduce_full<M>(v);
+  if (lane == 0) s_waves[w] = v;
+  __syncthreads();
+  typename M::value r = M::identity();
+  if (w == 0) {
+    r = (lane < nw) ? s_waves[lane] : M::identity();
+    r = wave::reduce_full<M>(r);
+  }
+  return r;
+}
+
+// How the final value is written.
+enum final_mode : int {
+  final_with_init = 0,     // *out = op(init, r)                (algorithms::reduce)
+  final_accumulate = 1,    // *out = op(*out, r)                (sycl::reduction)
+  final_identity_init = 2, // *out = r  (== op(identity, r))   (initialize_to_identity)
+};
+
+template <class T, class M, class OutT>
+struct params {
+  const T *a;
+  const T *b;         // second operand of a dot product
+  uint64_t head;      // scalar elements before the 16-byte aligned region
+  uint64_t nvec;      // 16-byte chunks in the aligned region
+  uint64_t nscalar;   // head + tail elements handled by the scalar loop
+  uint64_t tail_start;
+  typename M::value *partials;
+  sync_header *sync;
+  OutT *out;
+  typename elem_traits<T>::work init;
+  int mode;
+};
+
+template <class T>
+__device__ __forceinline__ u32x4 load16(const T *base, uint64_t chunk) {
+  return reinterpret_cast<const u32x4 *>(base)[chunk];
+}
+
+template <class T, class W, int VEC>
+__device__ __forceinline__ void unpack(u32x4 q, W (&out)[VEC]) {
+  using S = typename elem_traits<T>::storage;
+  S tmp[VEC];
+  __builtin_memcpy(tmp, &q, 16);
+#pragma unroll
+  for (int k = 0; k < VEC; ++k) out[k] = elem_traits<T>::to_work(__builtin_bit_cast(T, tmp[k]));
+}
+
+template <class T, class M, class Op, class OutT, int UNROLL, bool DOT>
+__global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
+  using W = typename elem_traits<T>::work;
+  using V = typename M::value;
+  constexpr int VEC = 16 / sizeof(T);
+
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+
+  V acc[UNROLL];
+#pragma unroll
+  for (int u = 0; u < UNROLL; ++u) acc[u] = M::identity();
+
+  const T *va = p.a + p.head;
+  const T *vb = DOT ? p.b + p.head : nullptr;
+  uint64_t c = gtid;
+  // Main loop: UNROLL independent 128-bit loads (per operand) in flight.
+  for (; c + (UNROLL - 1) * stride < p.nvec; c += UNROLL * stride) {
+    u32x4 qa[UNROLL];
+    u32x4 qb[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      qa[u] = load16(va, c + u * stride);
+      if constexpr (DOT) qb[u] = load16(vb, c + u * stride);
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      W xa[VEC];
+      unpack<T, W, VEC>(qa[u], xa);
+      if constexpr (DOT) {
+        W xb[VEC];
+        unpack<T, W, VEC>(qb[u], xb);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc[u] = M::insert2(acc[u], xa[k], xb[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc[u] = M::insert(acc[u], xa[k]);
+      }
+    }
+  }
+  for (; c < p.nvec; c += stride) {
+    W xa[VEC];
+    unpack<T, W, VEC>(load16(va, c), xa);
+    if constexpr (DOT) {
+      W xb[VEC];
+      unpack<T, W, VEC>(load16(vb, c), xb);
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) acc[0] = M::insert2(acc[0], xa[k], xb[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) acc[0] = M::insert(acc[0], xa[k]);
+    }
+  }
+  // Scalar elements: the misaligned head and the tail.
+  for (uint64_t s = gtid; s < p.nscalar; s += stride) {
+    const uint64_t idx = (s < p.head) ? s : p.tail_start + (s - p.head);
+    W xa = elem_traits<T>::to_work(p.a[idx]);
+    if constexpr (DOT) {
+      acc[0] = M::insert2(acc[0], xa, elem_traits<T>::to_work(p.b[idx]));
+    } else {
+      acc[0] = M::insert(acc[0], xa);
+    }
+  }
+  V v = acc[0];
+#pragma unroll
+  for (int u = 1; u < UNROLL; ++u) v = M::apply(v, acc[u]);
+
+  __shared__ V s_waves[max_waves];
+  __shared__ int s_last;
+  V bv = block_reduce<M>(v, s_waves);
+  if (threadIdx.x == 0) {
+    p.partials[blockIdx.x] = bv;
+    __threadfence();
+    uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+
+  // Last work-group: fold the partials in a fixed order.
+  const int nparts = gridDim.x;
+  if constexpr (M::compensated) {
+    // Cross-group combine of (sum, error) pairs in FP64.
+    __shared__ double s_dwaves[max_waves];
+    double d = -0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+      V q = p.partials[i];
+      d += static_cast<double>(q.s) + static_cast<double>(q.c);
+    }
+    d = block_reduce<double_plus<double>>(d, s_dwaves);
+    if (threadIdx.x == 0) {
+      double r = d;
+      if (p.mode == final_with_init) r = static_cast<double>(p.init) + d;
+      else if (p.mode == final_accumulate) r = static_cast<double>(*p.out) + d;
+      *p.out = static_cast<OutT>(r);
+      __hip_atomic_store(&p.sync->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else {
+    V r = M::identity();
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) r = M::apply(r, p.partials[i]);
+    __syncthreads();  // s_waves is reused
+    r = block_reduce<M>(r, s_waves);
+    if (threadIdx.x == 0) {
+      if (p.mode == final_with_init) {
+        r = Op::apply(p.init, r);
+      } else if (p.mode == final_accumulate) {
+        r = Op::apply(static_cast<W>(*p.out), r);
+      }
+      *p.out = static_cast<OutT>(r);
+      __hip_atomic_store(&p.sync->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+}  // namespace reduce_detail
+}  // namespace acpp_impl

@@ -1,0 +1,151 @@
+// Host launcher of the device-wide reduction (see reduce_common.hpp).
+#pragma once
+
+#include "launch_util.hpp"
+#include "reduce_common.hpp"
+
+namespace acpp_impl {
+namespace reduce_detail {
+
+// Element types whose 16-byte chunks are loaded with one instruction.
+template <class T>
+constexpr int vec_width() {
+  return 16 / static_cast<int>(sizeof(typename elem_traits<T>::storage));
+}
+
+template <class T, class M, class Op, class OutT, bool DOT>
+acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
+                   const T *b, uint64_t n, OutT *out,
+                   typename elem_traits<T>::work init, int mode,
+                   const acpp_launch_config *cfg, const char *name) {
+  using V = typename M::value;
+  constexpr int VEC = vec_width<T>();
+  int cus = 0;
+  acpp_status st = enter_device(g, &cus);
+  if (st != ACPP_SUCCESS) return st;
+
+  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 256;
+  if (!valid_work_group_size(wg))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT,
+                     std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
+  const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : 4;
+  if (unroll != 2 && unroll != 4 && unroll != 8)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT,
+                     std::string(name) + ": items_per_thread must be 2, 4 or 8");
+
+  // 16-byte aligned region; a dot product needs both operands equally
+  // misaligned, otherwise everything goes through the scalar loop.
+  const uintptr_t abytes = reinterpret_cast<uintptr_t>(a);
+  uint64_t head = ((16 - (abytes & 15)) & 15) / sizeof(typename elem_traits<T>::storage);
+  bool vec_ok = (abytes % sizeof(typename elem_traits<T>::storage)) == 0;
+  if (DOT) vec_ok = vec_ok && ((reinterpret_cast<uintptr_t>(b) & 15) == (abytes & 15));
+  if (!vec_ok || head > n) head = n;
+  const uint64_t nvec = (n - head) / VEC;
+  const uint64_t tail_start = head + nvec * VEC;
+  const uint64_t nscalar = head + (n - tail_start);
+
+  params<T, M, OutT> p;
+  p.a = a;
+  p.b = b;
+  p.head = head;
+  p.nvec = nvec;
+  p.nscalar = nscalar;
+  p.tail_start = tail_start;
+  p.out = out;
+  p.init = init;
+  p.mode = mode;
+
+  auto run = [&](auto kernel) -> acpp_status {
+    int resident = 1;
+    acpp_status s = occupancy(kernel, static_cast<int>(wg), &resident);
+    if (s != ACPP_SUCCESS) return s;
+    uint64_t max_groups = static_cast<uint64_t>(resident) * cus;
+    if (cfg && cfg->groups_per_cu) {
+      uint64_t req = static_cast<uint64_t>(cfg->groups_per_cu) * cus;
+      if (req < max_groups) max_groups = req;
+    }
+    uint64_t needed = ceil_div(nvec + nscalar, static_cast<uint64_t>(wg) * unroll);
+    uint64_t grid = needed < 1 ? 1 : needed;
+    if (grid > max_groups) grid = max_groups;
+
+    void *partials = nullptr;
+    s = group_obtain(g, grid * sizeof(V), &partials);
+    if (s != ACPP_SUCCESS) return s;
+    void *sync = nullptr;
+    size_t cap = 0;
+    s = group_obtain_sync(g, sizeof(sync_header), stream, &sync, &cap);
+    if (s != ACPP_SUCCESS) return s;
+    p.partials = static_cast<V *>(partials);
+    p.sync = static_cast<sync_header *>(sync);
+    hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(grid)), dim3(wg), 0, stream, p);
+    return check_launch(name);
+  };
+
+  // Only sums (the tuned hot path) get every unroll depth; other operators
+  // use 4 to bound the number of kernel instantiations.
+  if constexpr (Op::id == ops::op_plus) {
+    if (unroll == 2) return run(reduce_kernel<T, M, Op, OutT, 2, DOT>);
+    if (unroll == 8) return run(reduce_kernel<T, M, Op, OutT, 8, DOT>);
+  }
+  return run(reduce_kernel<T, M, Op, OutT, 4, DOT>);
+}
+
+// algorithms::reduce / sycl::reduction entry for element type T.
+template <class T, class OutT, class InitT>
+acpp_status reduce_entry(acpp_allocation_group *g, hipStream_t stream, const T *first,
+                         uint64_t n, OutT *out, InitT init, int op, int mode,
+                         const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  using E = std::conditional_t<std::is_floating_point_v<W>, W, T>;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": out is NULL");
+  if (n > 0 && !first) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": first is NULL");
+  if (n == 0 && mode == final_with_init) {
+    // numeric.hpp:165-169: empty input leaves *out untouched.
+    return with_op<W, E>(op, [&](auto) { return clear_error(); });
+  }
+  return with_op<W, E>(op, [&](auto opv) -> acpp_status {
+    using Op = decltype(opv);
+    acpp_status s = launch<T, plain_monoid<W, Op>, Op, OutT, false>(
+        g, static_cast<hipStream_t>(stream), first, nullptr, n, out, static_cast<W>(init), mode, cfg,
+        name);
+    return s == ACPP_SUCCESS ? clear_error() : s;
+  });
+}
+
+// Compensated sum (is_dot == false, b ignored) or dot product.
+template <class T>
+acpp_status compensated_entry(acpp_allocation_group *g, hipStream_t stream, const T *a,
+                              const T *b, bool is_dot, uint64_t n, float *out, float init,
+                              const acpp_launch_config *cfg, const char *name) {
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": out is NULL");
+  if (n > 0 && (!a || (is_dot && !b)))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": input is NULL");
+  if (n == 0) return clear_error();
+  acpp_status s;
+  using P = ops::plus<float, float>;
+  if (is_dot)
+    s = launch<T, comp_monoid, P, float, true>(g, stream, a, b, n, out, init, final_with_init, cfg, name);
+  else
+    s = launch<T, comp_monoid, P, float, false>(g, stream, a, nullptr, n, out, init, final_with_init, cfg, name);
+  return s == ACPP_SUCCESS ? clear_error() : s;
+}
+
+template <class T, class OutT>
+acpp_status dot_entry(acpp_allocation_group *g, hipStream_t stream, const T *a, uint64_t n,
+                      const T *b, OutT *out, typename elem_traits<T>::work init,
+                      const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": out is NULL");
+  if (n > 0 && (!a || !b)) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": input is NULL");
+  if (n == 0) return clear_error();
+  using P = ops::plus<W, W>;
+  acpp_status s = launch<T, plain_monoid<W, P>, P, OutT, true>(g, stream, a, b, n, out, init,
+                                                               final_with_init, cfg, name);
+  return s == ACPP_SUCCESS ? clear_error() : s;
+}
+
+}  // namespace reduce_detail
+}  // namespace acpp_impl

@@ -1,0 +1,282 @@
+// Host runtime of the library: error state, device queries and the scratch
+// allocation cache.
+//
+// The cache mirrors the reference's util::allocation_cache /
+// allocation_group (include/hipSYCL/algorithms/util/allocation_cache.hpp:42-181):
+// a thread-safe best-fit pool per device, and a non-thread-safe group that
+// returns everything it obtained to the pool when released.  It adds a second
+// pool of "sync regions" (zeroed once, kept reusable by the kernels), which
+// is what lets the scan and the reduction run as single launches.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace acpp_impl {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+acpp_status set_error(acpp_status code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+acpp_status clear_error() {
+  g_last_error.clear();
+  return ACPP_SUCCESS;
+}
+
+acpp_status hip_error(hipError_t e, const char *what) {
+  std::string msg = std::string(what) + ": " + hipGetErrorName(e) + " (" +
+                    hipGetErrorString(e) + ")";
+  if (e == hipErrorOutOfMemory) return set_error(ACPP_ERROR_OUT_OF_MEMORY, msg);
+  return set_error(ACPP_ERROR_HIP, msg);
+}
+
+acpp_status check_launch(const char *kernel_name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_error(e, kernel_name);
+  return ACPP_SUCCESS;
+}
+
+namespace {
+struct device_info {
+  int compute_units = 0;
+};
+std::mutex g_dev_mutex;
+std::vector<device_info> g_devices;
+
+acpp_status query_compute_units(int dev, int *out) {
+  std::lock_guard<std::mutex> lock{g_dev_mutex};
+  if (dev < 0) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "negative device id");
+  if (static_cast<size_t>(dev) >= g_devices.size()) g_devices.resize(dev + 1);
+  if (g_devices[dev].compute_units == 0) {
+    int cu = 0;
+    ACPP_CHECK_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+    g_devices[dev].compute_units = cu;
+  }
+  *out = g_devices[dev].compute_units;
+  return ACPP_SUCCESS;
+}
+}  // namespace
+
+}  // namespace acpp_impl
+
+using namespace acpp_impl;
+
+namespace {
+struct allocation {
+  void *ptr = nullptr;
+  size_t size = 0;
+  int dev = 0;
+  bool sync = false;
+};
+}  // namespace
+
+struct acpp_allocation_cache {
+  int type = ACPP_ALLOCATION_DEVICE;
+  std::mutex mutex;
+  std::vector<allocation> available;
+
+  acpp_status allocate(size_t bytes, int dev, allocation &out) {
+    int prev = 0;
+    ACPP_CHECK_HIP(hipGetDevice(&prev));
+    if (prev != dev) ACPP_CHECK_HIP(hipSetDevice(dev));
+    void *p = nullptr;
+    hipError_t e;
+    if (type == ACPP_ALLOCATION_SHARED)
+      e = hipMallocManaged(&p, bytes);
+    else if (type == ACPP_ALLOCATION_HOST)
+      e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    else
+      e = hipMalloc(&p, bytes);
+    if (prev != dev) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return hip_error(e, "scratch allocation");
+    out.ptr = p;
+    out.size = bytes;
+    out.dev = dev;
+    return ACPP_SUCCESS;
+  }
+
+  void deallocate(const allocation &a) {
+    if (type == ACPP_ALLOCATION_HOST)
+      (void)hipHostFree(a.ptr);
+    else
+      (void)hipFree(a.ptr);
+  }
+
+  // Best fit: the smallest available allocation that is large enough and
+  // suitably aligned (allocation_cache.hpp:82-114).
+  bool find(size_t bytes, size_t alignment, int dev, bool sync, allocation &out) {
+    std::lock_guard<std::mutex> lock{mutex};
+    size_t best = available.size();
+    for (size_t i = 0; i < available.size(); ++i) {
+      const allocation &a = available[i];
+      if (a.dev != dev || a.sync != sync || a.size < bytes) continue;
+      if (reinterpret_cast<uintptr_t>(a.ptr) % alignment != 0) continue;
+      if (best == available.size() || a.size < available[best].size) best = i;
+    }
+    if (best == available.size()) return false;
+    out = available[best];
+    available.erase(available.begin() + best);
+    return true;
+  }
+
+  void give_back(const allocation &a) {
+    std::lock_guard<std::mutex> lock{mutex};
+    available.push_back(a);
+  }
+
+  void purge() {
+    std::lock_guard<std::mutex> lock{mutex};
+    for (const allocation &a : available) deallocate(a);
+    available.clear();
+  }
+};
+
+struct acpp_allocation_group {
+  acpp_allocation_cache *parent = nullptr;
+  int dev = 0;
+  std::vector<allocation> held;
+};
+
+namespace {
+// Round scratch requests up so that regions are reused across nearby sizes.
+size_t round_request(size_t bytes) {
+  size_t r = 256;
+  while (r < bytes) r <<= 1;
+  return r;
+}
+}  // namespace
+
+namespace acpp_impl {
+
+acpp_status group_obtain(acpp_allocation_group *g, size_t bytes, void **out) {
+  allocation a;
+  bytes = round_request(bytes);
+  if (!g->parent->find(bytes, 256, g->dev, false, a)) {
+    acpp_status s = g->parent->allocate(bytes, g->dev, a);
+    if (s != ACPP_SUCCESS) return s;
+  }
+  g->held.push_back(a);
+  *out = a.ptr;
+  return ACPP_SUCCESS;
+}
+
+acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
+                              hipStream_t stream, void **out,
+                              size_t *capacity_bytes) {
+  if (g->parent->type != ACPP_ALLOCATION_DEVICE && g->parent->type != ACPP_ALLOCATION_SHARED)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT,
+                     "kernels need device-accessible scratch; use an "
+                     "ACPP_ALLOCATION_DEVICE or _SHARED cache");
+  allocation a;
+  bytes = round_request(bytes);
+  if (!g->parent->find(bytes, 256, g->dev, true, a)) {
+    acpp_status s = g->parent->allocate(bytes, g->dev, a);
+    if (s != ACPP_SUCCESS) return s;
+    a.sync = true;
+    hipError_t e = hipMemsetAsync(a.ptr, 0, a.size, stream);
+    if (e != hipSuccess) {
+      g->parent->deallocate(a);
+      return hip_error(e, "zeroing a new sync region");
+    }
+  }
+  g->held.push_back(a);
+  *out = a.ptr;
+  *capacity_bytes = a.size;
+  return ACPP_SUCCESS;
+}
+
+int group_device(const acpp_allocation_group *g) { return g->dev; }
+
+acpp_status enter_device(acpp_allocation_group *g, int *compute_units) {
+  int cur = 0;
+  ACPP_CHECK_HIP(hipGetDevice(&cur));
+  if (cur != g->dev) ACPP_CHECK_HIP(hipSetDevice(g->dev));
+  return query_compute_units(g->dev, compute_units);
+}
+
+}  // namespace acpp_impl
+
+extern "C" {
+
+const char *acpp_last_error_message(void) { return g_last_error.c_str(); }
+
+const char *acpp_version(void) { return "0.1.0"; }
+
+acpp_status acpp_device_compute_units(int device, int *out) {
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "out is NULL");
+  return query_compute_units(device, out);
+}
+
+acpp_status acpp_allocation_cache_create(int allocation_type, acpp_allocation_cache **out) {
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "out is NULL");
+  if (allocation_type < ACPP_ALLOCATION_DEVICE || allocation_type > ACPP_ALLOCATION_HOST)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, "unknown allocation type");
+  auto *c = new (std::nothrow) acpp_allocation_cache;
+  if (!c) return set_error(ACPP_ERROR_OUT_OF_MEMORY, "host allocation failed");
+  c->type = allocation_type;
+  *out = c;
+  return clear_error();
+}
+
+acpp_status acpp_allocation_cache_purge(acpp_allocation_cache *cache) {
+  if (!cache) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "cache is NULL");
+  cache->purge();
+  return clear_error();
+}
+
+acpp_status acpp_allocation_cache_destroy(acpp_allocation_cache *cache) {
+  if (!cache) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "cache is NULL");
+  cache->purge();
+  delete cache;
+  return clear_error();
+}
+
+acpp_status acpp_allocation_group_create(acpp_allocation_cache *cache, int device,
+                                         acpp_allocation_group **out) {
+  if (!cache || !out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "cache or out is NULL");
+  if (device < 0) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "negative device id");
+  auto *g = new (std::nothrow) acpp_allocation_group;
+  if (!g) return set_error(ACPP_ERROR_OUT_OF_MEMORY, "host allocation failed");
+  g->parent = cache;
+  g->dev = device;
+  *out = g;
+  return clear_error();
+}
+
+acpp_status acpp_allocation_group_obtain(acpp_allocation_group *group, size_t bytes,
+                                         size_t alignment, void **out) {
+  if (!group || !out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "group or out is NULL");
+  if (alignment == 0 || (alignment & (alignment - 1)) != 0 || alignment > 256)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, "alignment must be a power of two <= 256");
+  allocation a;
+  size_t req = round_request(bytes);
+  if (!group->parent->find(req, alignment, group->dev, false, a)) {
+    acpp_status s = group->parent->allocate(req, group->dev, a);
+    if (s != ACPP_SUCCESS) return s;
+  }
+  group->held.push_back(a);
+  *out = a.ptr;
+  return clear_error();
+}
+
+acpp_status acpp_allocation_group_release(acpp_allocation_group *group) {
+  if (!group) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "group is NULL");
+  for (const allocation &a : group->held) group->parent->give_back(a);
+  group->held.clear();
+  return clear_error();
+}
+
+acpp_status acpp_allocation_group_destroy(acpp_allocation_group *group) {
+  if (!group) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "group is NULL");
+  acpp_allocation_group_release(group);
+  delete group;
+  return clear_error();
+}
+
+}  // extern "C"

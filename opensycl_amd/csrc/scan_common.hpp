@@ -1,0 +1,291 @@
+// Single-pass decoupled look-back scan for gfx950.
+//
+// Reference behaviour being replaced
+// (include/hipSYCL/algorithms/scan/decoupled_lookback_scan.hpp):
+//   - a status-initialisation kernel (584-589), then the scan kernel (633-642);
+//   - work-groups of 128 work-items x 16 invocations; each invocation is a
+//     Kogge-Stone scan in LDS with 2 barriers per step plus a broadcast
+//     (49-72, 122-135, 181-217): ~277 barriers per 2048-element tile;
+//   - dynamic tile ids from an atomic counter (379-391) so that every tile's
+//     predecessors have started;
+//   - per-tile status {invalid, aggregate, prefix} published with acq_rel
+//     atomics next to separate value arrays (30-47, 407-447), and a serial
+//     single-lane look-back (219-257) combining op(earlier, later).
+//
+// MI355X design (one launch, no initialisation kernel):
+//   - a tile is B work-items x R rows x one 16-byte vector per work-item per
+//     row; the R loads per work-item are issued back to back (coalesced
+//     128-bit loads, R x 16 bytes in flight per lane);
+//   - each work-item scans its vector in registers, each wave scans the
+//     per-lane totals with DPP, and one LDS exchange of the wave totals
+//     (one barrier) gives every work-item its offset inside the tile;
+//   - the status of a tile is self-validating: value bits and a tag
+//     (epoch << 2 | flag) share one 64-bit word (two words for 64-bit
+//     values, each carrying the tag), written and read with relaxed
+//     device-scope atomics.  Words from earlier launches carry an older
+//     epoch and read as "invalid", so nothing has to be cleared;
+//   - the look-back is done by a whole wave: 64 predecessors are polled at
+//     once, the nearest "prefix" among them is found with a ballot, and the
+//     window is reduced with DPP;
+//   - the last work-group to finish resets the tile ticket and the done
+//     counter and advances the epoch.
+#pragma once
+
+#include "internal.hpp"
+
+namespace acpp_impl {
+namespace scan_detail {
+
+constexpr int max_waves = 16;
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+enum : uint32_t { st_invalid = 0, st_aggregate = 1, st_prefix = 2 };
+
+__device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p) {
+  return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_u64(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Tile status encoding.
+template <class W, int Bytes = sizeof(W)>
+struct status_io;
+
+template <class W>
+struct status_io<W, 4> {
+  static constexpr int words = 1;
+  __device__ __forceinline__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch,
+                                                 uint32_t flag, W v) {
+    const uint64_t tag = (static_cast<uint64_t>(epoch) << 2) | flag;
+    atomic_store_u64(st + tile, (tag << 32) | __builtin_bit_cast(uint32_t, v));
+  }
+  __device__ __forceinline__ static uint32_t read(const uint64_t *st, uint32_t tile,
+                                                  uint32_t epoch, W &v) {
+    const uint64_t w = atomic_load_u64(st + tile);
+    const uint32_t tag = static_cast<uint32_t>(w >> 32);
+    if ((tag >> 2) != epoch) return st_invalid;
+    v = __builtin_bit_cast(W, static_cast<uint32_t>(w));
+    return tag & 3u;
+  }
+};
+
+template <class W>
+struct status_io<W, 8> {
+  static constexpr int words = 2;
+  __device__ __forceinline__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch,
+                                                 uint32_t flag, W v) {
+    const uint64_t tag = (static_cast<uint64_t>(epoch) << 2) | flag;
+    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+    atomic_store_u64(st + 2 * static_cast<uint64_t>(tile), (tag << 32) | (bits & 0xffffffffu));
+    atomic_store_u64(st + 2 * static_cast<uint64_t>(tile) + 1, (tag << 32) | (bits >> 32));
+  }
+  __device__ __forceinline__ static uint32_t read(const uint64_t *st, uint32_t tile,
+                                                  uint32_t epoch, W &v) {
+    const uint64_t w0 = atomic_load_u64(st + 2 * static_cast<uint64_t>(tile));
+    const uint64_t w1 = atomic_load_u64(st + 2 * static_cast<uint64_t>(tile) + 1);
+    const uint32_t t0 = static_cast<uint32_t>(w0 >> 32);
+    const uint32_t t1 = static_cast<uint32_t>(w1 >> 32);
+    // Both halves must come from the same publication.
+    if (t0 != t1 || (t0 >> 2) != epoch) return st_invalid;
+    v = __builtin_bit_cast(W, (w1 << 32) | (w0 & 0xffffffffu));
+    return t0 & 3u;
+  }
+};
+
+// Exclusive prefix of `tile`: op over all predecessors, found by a wave-wide
+// look-back.  Must be called by a whole wave; the result is uniform.
+template <class Op, class W>
+__device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch) {
+  using SIO = status_io<W>;
+  const int lane = wave::lane_id();
+  W excl = Op::identity();
+  int64_t base = static_cast<int64_t>(tile) - 1;
+  for (;;) {
+    const int64_t j = base - lane;  // lane 0 = nearest predecessor
+    W val = Op::identity();
+    int p;
+    for (;;) {
+      uint32_t flag = st_prefix;  // positions before tile 0 act as an empty prefix
+      if (j >= 0) flag = SIO::read(status, static_cast<uint32_t>(j), epoch, val);
+      if (j < 0) val = Op::identity();
+      const uint64_t pm = wave::ballot(flag == st_prefix);
+      p = pm ? __builtin_ctzll(pm) : 64;
+      const uint64_t need = (p >= 63) ? ~0ull : ((2ull << p) - 1);
+      const uint64_t bad = wave::ballot(flag == st_invalid) & need;
+      if (bad == 0) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    const W x = (lane <= p) ? val : Op::identity();
+    const W window = wave::reduce_full<Op>(x);
+    excl = Op::apply(window, excl);
+    if (p < 64) return excl;
+    base -= 64;
+  }
+}
+
+template <class T>
+struct params {
+  using W = typename elem_traits<T>::work;
+  const T *in;
+  T *out;
+  uint64_t n;
+  W init;
+  int has_init;
+  int exclusive;
+  int vec_io;
+  uint32_t num_tiles;
+  uint64_t status_words;  // capacity of the status array, in 64-bit words
+  sync_header *sync;
+  uint64_t *status;
+};
+
+template <class T, class Op, int R>
+__global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
+  using W = typename elem_traits<T>::work;
+  using SIO = status_io<W>;
+  constexpr int VEC = 16 / sizeof(T);
+
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int nw = B >> 6;
+  const int w = tid >> 6;
+  const int lane = tid & 63;
+
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_epoch;
+  __shared__ W s_wave[R][max_waves];
+  __shared__ W s_excl;
+
+  if (tid == 0) {
+    s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    s_epoch = __hip_atomic_load(&p.sync->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+  }
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t epoch = s_epoch;
+
+  const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
+  const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+  const bool full = p.vec_io && (base + tile_elems <= p.n);
+
+  W v[R][VEC];
+  if (full) {
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
+    u32x4 q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      T tmp[VEC];
+      __builtin_memcpy(tmp, &q[r], 16);
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
+        v[r][k] = (idx < p.n) ? elem_traits<T>::to_work(p.in[idx]) : Op::identity();
+      }
+    }
+  }
+
+  // Work-item-local inclusive scan of each row vector.
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int k = 1; k < VEC; ++k) v[r][k] = Op::apply(v[r][k - 1], v[r][k]);
+  }
+  // Wave scans of the per-lane row totals.
+  W wexcl[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const W incl = wave::inclusive_scan<Op>(v[r][VEC - 1]);
+    wexcl[r] = wave::shift_up<Op>(incl);
+    if (lane == 63) s_wave[r][w] = incl;
+  }
+  __syncthreads();
+  // Offsets of (row, wave) inside the tile, in element order.
+  W before[R];
+  W carry = Op::identity();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    before[r] = Op::identity();
+    for (int ww = 0; ww < nw; ++ww) {
+      const W x = s_wave[r][ww];
+      if (ww == w) before[r] = carry;
+      carry = Op::apply(carry, x);
+    }
+  }
+  const W agg = carry;
+
+  W tile_prefix;
+  if (tile == 0) {
+    tile_prefix = p.has_init ? p.init : Op::identity();
+    if (tid == 0 && p.num_tiles > 1)
+      SIO::publish(p.status, 0, epoch, st_prefix, Op::apply(tile_prefix, agg));
+  } else {
+    if (tid == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
+    if (w == 0) {
+      const W excl = lookback<Op, W>(p.status, tile, epoch);
+      if (lane == 0) {
+        if (tile + 1 < p.num_tiles)
+          SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(excl, agg));
+        s_excl = excl;
+      }
+    }
+    __syncthreads();
+    tile_prefix = s_excl;
+  }
+
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const W P = Op::apply(tile_prefix, Op::apply(before[r], wexcl[r]));
+    T o[VEC];
+    if (p.exclusive) {
+      o[0] = static_cast<T>(P);
+#pragma unroll
+      for (int k = 1; k < VEC; ++k) o[k] = static_cast<T>(Op::apply(P, v[r][k - 1]));
+    } else {
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) o[k] = static_cast<T>(Op::apply(P, v[r][k]));
+    }
+    if (full) {
+      u32x4 q;
+      __builtin_memcpy(&q, o, 16);
+      reinterpret_cast<u32x4 *>(p.out + base)[r * B + tid] = q;
+    } else {
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
+        if (idx < p.n) p.out[idx] = o[k];
+      }
+    }
+  }
+
+  // The last work-group to finish makes the sync region reusable.
+  if (tid == 0) {
+    const uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == p.num_tiles - 1) {
+      __hip_atomic_store(&p.sync->tile_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&p.sync->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t next = epoch;
+      if (epoch >= max_epoch) {
+        // Epoch space exhausted (once per 2^30 launches): clear every status
+        // word so that epoch 1 can be reused.
+        for (uint64_t i = 0; i < p.status_words; ++i) atomic_store_u64(p.status + i, 0);
+        next = 0;
+      }
+      __hip_atomic_store(&p.sync->epoch, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+}  // namespace scan_detail
+}  // namespace acpp_impl

@@ -1,0 +1,98 @@
+// Host launcher of the decoupled look-back scan (see scan_common.hpp).
+#pragma once
+
+#include "launch_util.hpp"
+#include "scan_common.hpp"
+
+namespace acpp_impl {
+namespace scan_detail {
+
+// Default rows (16-byte vectors) per work-item: 8 for 4/8-byte elements,
+// fewer for narrow types whose 16-byte vector holds 8 or 16 elements.
+template <class T>
+constexpr int default_rows() {
+  return sizeof(T) >= 4 ? 8 : (sizeof(T) == 2 ? 4 : 2);
+}
+
+template <class T, class Op>
+acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, uint64_t n,
+                   T *out, bool exclusive, bool has_init, typename elem_traits<T>::work init,
+                   const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  constexpr int VEC = 16 / sizeof(T);
+  int cus = 0;
+  acpp_status st = enter_device(g, &cus);
+  if (st != ACPP_SUCCESS) return st;
+
+  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 256;
+  if (!valid_work_group_size(wg))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT,
+                     std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
+  int rows = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows<T>();
+  constexpr bool tunable = (Op::id == ops::op_plus) && sizeof(T) >= 4;
+  if (rows != default_rows<T>()) {
+    if (!tunable || (rows != 2 && rows != 4 && rows != 8))
+      return set_error(ACPP_ERROR_INVALID_ARGUMENT,
+                       std::string(name) + ": items_per_thread (rows per work-item) must be " +
+                           std::to_string(default_rows<T>()) +
+                           (tunable ? ", 2 or 4" : " for this element type and operator"));
+  }
+
+  const uint64_t tile_elems = static_cast<uint64_t>(wg) * rows * VEC;
+  const uint64_t num_tiles = ceil_div(n, tile_elems);
+  if (num_tiles >= (1ull << 31))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": problem too large");
+
+  using SIO = status_io<W>;
+  const uint64_t status_bytes = num_tiles * SIO::words * sizeof(uint64_t);
+  void *region = nullptr;
+  size_t cap = 0;
+  st = group_obtain_sync(g, sizeof(sync_header) + status_bytes, stream, &region, &cap);
+  if (st != ACPP_SUCCESS) return st;
+
+  params<T> p;
+  p.in = in;
+  p.out = out;
+  p.n = n;
+  p.init = init;
+  p.has_init = has_init ? 1 : 0;
+  p.exclusive = exclusive ? 1 : 0;
+  p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  p.num_tiles = static_cast<uint32_t>(num_tiles);
+  p.status_words = (cap - sizeof(sync_header)) / sizeof(uint64_t);
+  p.sync = static_cast<sync_header *>(region);
+  p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
+
+  auto run = [&](auto kernel) -> acpp_status {
+    hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(num_tiles)), dim3(wg), 0, stream, p);
+    return check_launch(name);
+  };
+  if constexpr (tunable) {
+    if (rows == 2) return run(scan_kernel<T, Op, 2>);
+    if (rows == 4) return run(scan_kernel<T, Op, 4>);
+  }
+  return run(scan_kernel<T, Op, default_rows<T>()>);
+}
+
+template <class T>
+acpp_status scan_entry(acpp_allocation_group *g, hipStream_t stream, const T *first, uint64_t n,
+                       T *d_first, int op, bool exclusive, const T *init_ptr, T init_value,
+                       const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (n > 0 && (!first || !d_first))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": first or d_first is NULL");
+  bool has_init = exclusive || init_ptr != nullptr;
+  T init = exclusive ? init_value : (init_ptr ? *init_ptr : T{});
+  return with_op<W, T>(op, [&](auto opv) -> acpp_status {
+    using Op = decltype(opv);
+    // decoupled_lookback_scan.hpp:565-566: an empty problem enqueues nothing.
+    if (n == 0) return clear_error();
+    acpp_status s = launch<T, Op>(g, stream, first, n, d_first, exclusive, has_init,
+                                  elem_traits<T>::to_work(init), cfg, name);
+    return s == ACPP_SUCCESS ? clear_error() : s;
+  });
+}
+
+}  // namespace scan_detail
+}  // namespace acpp_impl

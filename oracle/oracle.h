@@ -1,0 +1,97 @@
+/*
+ * oracle.h -- CPU restatement of the reference's algorithms for the
+ * device-wide primitives.  TEST INFRASTRUCTURE ONLY: this library is the
+ * checker for the HIP kernels (tests, smoke check, CPU baseline timing) and
+ * is never linked into, loaded by or called from the product library.
+ *
+ * Two families of functions:
+ *   oracle_<op>_<t>            serial semantics that the reference's own tests
+ *                              assert against (std::accumulate /
+ *                              std::inclusive_scan / serial atomic folds);
+ *   oracle_*_omp_order_<t>     the order in which the reference's OpenMP host
+ *                              backend evaluates the same call (multi-threaded
+ *                              with OpenMP); used for floating-point order
+ *                              studies and as the timed CPU baseline.
+ * Element type tags: i8 i16 i32 i64 u8 u16 u32 u64 f32 f64.
+ * Operator codes equal __acpp_sscp_algorithm_op
+ * (include/hipSYCL/sycl/libkernel/sscp/builtins/builtin_config.hpp:27-37).
+ * Every function returns 0 on success and -1 on an invalid operator.
+ */
+#ifndef ACPP_ORACLE_H
+#define ACPP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORACLE_DECLARE(t, T)                                                                      \
+  int oracle_identity_##t(int op, T *out);                                                        \
+  int oracle_apply_##t(int op, T a, T b, T *out);                                                 \
+  int oracle_reduce_##t(const T *x, uint64_t n, T init, int op, T *out);                          \
+  int oracle_reduction_##t(const T *x, uint64_t n, T *out, int op, int initialize_to_identity);   \
+  int oracle_inclusive_scan_##t(const T *x, uint64_t n, T *y, int op, const T *init);             \
+  int oracle_exclusive_scan_##t(const T *x, uint64_t n, T *y, T init, int op);                    \
+  int oracle_dot_##t(const T *a, const T *b, uint64_t n, T init, T *out);                         \
+  int oracle_reduce_omp_order_##t(const T *x, uint64_t n, T init, int op, T *out,                 \
+                                  int num_threads);                                               \
+  int oracle_dot_omp_order_##t(const T *a, const T *b, uint64_t n, T init, T *out,                \
+                               int num_threads);                                                  \
+  int oracle_scan_omp_order_##t(const T *x, uint64_t n, T *y, int exclusive, int has_init,        \
+                                T init, int op, int num_threads);                                 \
+  int oracle_group_reduce_##t(const T *x, uint64_t n, uint64_t local_size, int op, T *y);         \
+  int oracle_group_inclusive_scan_##t(const T *x, uint64_t n, uint64_t local_size, int op, T *y); \
+  int oracle_group_exclusive_scan_##t(const T *x, uint64_t n, uint64_t local_size, T init,        \
+                                      int op, T *y);                                              \
+  int oracle_atomic_fold_##t(int op, const T *operands, uint64_t n, T initial, T *final_value);   \
+  int oracle_segmented_inclusive_scan_##t(const T *x, const uint64_t *offsets,                    \
+                                          uint64_t num_segments, T *y, int op);                   \
+  int oracle_segmented_reduce_##t(const T *x, const uint64_t *offsets, uint64_t num_segments,     \
+                                  T init, int op, T *out);
+
+ORACLE_DECLARE(i8, int8_t)
+ORACLE_DECLARE(i16, int16_t)
+ORACLE_DECLARE(i32, int32_t)
+ORACLE_DECLARE(i64, int64_t)
+ORACLE_DECLARE(u8, uint8_t)
+ORACLE_DECLARE(u16, uint16_t)
+ORACLE_DECLARE(u32, uint32_t)
+ORACLE_DECLARE(u64, uint64_t)
+ORACLE_DECLARE(f32, float)
+ORACLE_DECLARE(f64, double)
+
+/* FP64 oracles for floating-point sums: Neumaier-compensated FP64
+   accumulation (error ~ n * 2^-106 relative to the magnitude sum).
+   bf16 inputs are raw uint16 bit patterns; widening to float is exact. */
+double oracle_sum_f32_fp64(const float *x, uint64_t n);
+double oracle_sum_bf16_fp64(const uint16_t *x, uint64_t n);
+double oracle_dot_f32_fp64(const float *a, const float *b, uint64_t n);
+double oracle_dot_bf16_fp64(const uint16_t *a, const uint16_t *b, uint64_t n);
+double oracle_abs_sum_f32_fp64(const float *x, uint64_t n);
+int oracle_segmented_reduce_f32_fp64(const float *x, const uint64_t *offsets,
+                                     uint64_t num_segments, double *out);
+
+/* Histogram by fetch_add(1): bins[keys[i]] += 1 for keys[i] < num_bins. */
+int oracle_histogram_u32(const uint32_t *keys, uint64_t n, uint32_t *bins, uint32_t num_bins,
+                         int num_threads);
+/* slots[i % num_slots] = max(slots[i % num_slots], values[i]). */
+int oracle_fetch_max_strided_i32(const int32_t *values, uint64_t n, int32_t *slots,
+                                 uint32_t num_slots, int num_threads);
+/* Generic indexed fetch op: target[idx[i]] = op(target[idx[i]], values[i]). */
+int oracle_fetch_op_indexed_i32(int op, const int32_t *values, const uint32_t *idx, uint64_t n,
+                                int32_t *target, uint32_t num_targets);
+
+/* Number of OpenMP threads a parallel region would use by default. */
+int oracle_max_threads(void);
+
+/* Deliberate-mistake switch for the tests that show the oracle's pinning
+   checks catch plausible errors.  0 = faithful (default). */
+void oracle_set_fault(int fault);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACPP_ORACLE_H */

@@ -1,0 +1,144 @@
+"""GPU parity: inclusive_scan / exclusive_scan against the oracle.
+
+Integer scans must be bit-exact for every operator, size, alignment and
+launch configuration; floating-point sums are compared with an FP64 prefix
+sum within the bound stated in the test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import opensycl_amd as acpp
+import oracle as O
+from gpu_util import ALL_OPS, INT_TYPES, dev, host, op_input, rand
+
+pytestmark = pytest.mark.gpu
+
+# size ladder of tests/pstl/inclusive_scan.cpp:158-176 plus tile boundaries
+# (a tile is 256 work-items x 8 rows x 16 bytes = 8192 int32)
+SIZES = [1, 2, 127, 1000, 8191, 8192, 8193, 100003, 1 << 20]
+
+
+def _scan_case(torch, q, scratch, x, op, kind, init=None, config=None):
+    xd = dev(torch, x)
+    yd = torch.empty_like(xd)
+    if kind == "inclusive":
+        acpp.inclusive_scan(q, scratch, xd, yd, op, init, config=config)
+    else:
+        acpp.exclusive_scan(q, scratch, xd, yd, init, op, config=config)
+    q.wait()
+    return host(yd)
+
+
+@pytest.mark.parametrize("dtype", INT_TYPES)
+def test_scan_int_all_ops_exact(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    for op in ALL_OPS:
+        for n in SIZES:
+            x = op_input(dtype, op, n, seed=n * 7 + op)
+            init = dtype(3)
+            got = _scan_case(torch, q, scratch, x, op, "inclusive")
+            assert np.array_equal(got, O.inclusive_scan(x, op)), (dtype, op, n, "incl")
+            got = _scan_case(torch, q, scratch, x, op, "inclusive", init)
+            assert np.array_equal(got, O.inclusive_scan(x, op, init)), (dtype, op, n, "incl+init")
+            got = _scan_case(torch, q, scratch, x, op, "exclusive", init)
+            assert np.array_equal(got, O.exclusive_scan(x, init, op)), (dtype, op, n, "excl")
+
+
+def test_pstl_ladder_closed_forms(gpu, q, scratch):
+    # tests/pstl/inclusive_scan.cpp, exclusive_scan.cpp: iota input, plus with
+    # init 3 and multiplication with init 3.
+    torch, _ = gpu
+    for n in (1, 127, 1000, 1 << 20):
+        i = np.arange(n, dtype=np.int64)
+        x = i.astype(np.int32)
+        inc = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive", np.int32(3))
+        exc = _scan_case(torch, q, scratch, x, acpp.PLUS, "exclusive", np.int32(3))
+        assert np.array_equal(inc, ((i * (i + 1) // 2 + 3) & 0xffffffff).astype(np.uint32).view(np.int32))
+        assert np.array_equal(exc, ((i * (i - 1) // 2 + 3) & 0xffffffff).astype(np.uint32).view(np.int32))
+        m = _scan_case(torch, q, scratch, i.astype(np.uint64), acpp.MULTIPLY, "inclusive", np.uint64(3))
+        assert np.all(m == 0)
+        me = _scan_case(torch, q, scratch, i.astype(np.uint64), acpp.MULTIPLY, "exclusive", np.uint64(3))
+        assert me[0] == 3 and np.all(me[1:] == 0)
+
+
+@pytest.mark.parametrize("cfg", [(64, 2), (128, 4), (256, 2), (256, 4), (512, 8), (1024, 8),
+                                 (1024, 2)])
+def test_scan_launch_configs(gpu, q, scratch, cfg):
+    torch, _ = gpu
+    for dtype in (np.int32, np.int64, np.float32):
+        x = rand(dtype, 3 * 1024 * 1024 + 17, seed=cfg[0] + cfg[1])
+        if dtype == np.float32:
+            x = np.round(x * 100).astype(np.float32)  # small integers: fp sums are exact
+        got = _scan_case(torch, q, scratch, x, acpp.PLUS, "exclusive", dtype(1), config=cfg)
+        assert np.array_equal(got, O.exclusive_scan(x, dtype(1))), (dtype, cfg)
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_scan_misaligned_and_in_place(gpu, q, scratch, offset):
+    torch, _ = gpu
+    for dtype in (np.int8, np.int16, np.int32, np.int64):
+        x = rand(dtype, 100000 + offset, seed=offset)
+        xd = dev(torch, x)[offset:]
+        yd = torch.empty(100000, dtype=xd.dtype, device="cuda")
+        acpp.inclusive_scan(q, scratch, xd, yd)
+        q.wait()
+        assert np.array_equal(host(yd), O.inclusive_scan(x[offset:]))
+        # in place (d_first == first)
+        zd = dev(torch, x)
+        acpp.exclusive_scan(q, scratch, zd, zd, dtype(4))
+        q.wait()
+        assert np.array_equal(host(zd), O.exclusive_scan(x, dtype(4)))
+
+
+def test_scan_repeated_launches_share_scratch(gpu, q, scratch):
+    # the sync region is reused launch after launch (epochs advance); sizes
+    # alternate so stale words from a larger launch are present.
+    torch, _ = gpu
+    for rep in range(12):
+        n = [5000000, 70000, 1 << 20, 123][rep % 4]
+        x = rand(np.int32, n, seed=rep)
+        got = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive")
+        assert np.array_equal(got, O.inclusive_scan(x)), rep
+
+
+@pytest.mark.parametrize("dtype,eps", [(np.float32, 2.0 ** -24), (np.float64, 2.0 ** -53)])
+def test_scan_fp_sum_vs_fp64(gpu, q, scratch, dtype, eps):
+    torch, _ = gpu
+    for n in (1, 1000, 100003, 1 << 21):
+        x = rand(dtype, n, seed=n)
+        got = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive").astype(np.float64)
+        x64 = x.astype(np.float64)
+        exact = np.cumsum(x64)
+        depth = 64 + 2 * int(math.log2(max(n, 2)))
+        bound = depth * eps * np.cumsum(np.abs(x64)) + 1e-300
+        # np.cumsum in FP64 differs from the exact prefix by far less than eps(f32)
+        assert np.all(np.abs(got - exact) <= bound)
+
+
+def test_scan_fp_minmax_exact(gpu, q, scratch):
+    torch, _ = gpu
+    for dtype in (np.float32, np.float64):
+        x = rand(dtype, 200001, seed=5)
+        for op in (acpp.MIN, acpp.MAX):
+            got = _scan_case(torch, q, scratch, x, op, "inclusive")
+            assert np.array_equal(got, O.inclusive_scan(x, op))
+
+
+def test_scan_large_int32_exact(gpu, q, scratch):
+    # 2^28 int32 in [-2^15, 2^15] (the bandwidth-scale workload), exclusive, init 0
+    torch, _ = gpu
+    n = 1 << 28
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    xd = torch.randint(-(1 << 15), (1 << 15) + 1, (n,), dtype=torch.int32, device="cuda",
+                       generator=g)
+    yd = torch.empty_like(xd)
+    acpp.exclusive_scan(q, scratch, xd, yd, np.int32(0))
+    q.wait()
+    x = host(xd)
+    del xd
+    y = host(yd)
+    del yd
+    assert np.array_equal(y, O.exclusive_scan(x, np.int32(0)))
