@@ -84,8 +84,15 @@ typedef struct acpp_launch_config {
                                 (2, 4 or 8); scan: 16-byte rows per thread
                                 (2, 4 or 8) */
   uint32_t groups_per_cu;    /* reduce only: resident work-groups per CU */
-  uint32_t flags;            /* reserved, 0 */
+  uint32_t flags;            /* variant bits, 0 = default:
+                                ACPP_VARIANT_WAVE_LOOKBACK (scans) looks back
+                                with one wave instead of the whole group;
+                                ACPP_VARIANT_NONTEMPORAL (reduce) streams the
+                                input with non-temporal loads */
 } acpp_launch_config;
+
+#define ACPP_VARIANT_WAVE_LOOKBACK 1u
+#define ACPP_VARIANT_NONTEMPORAL 2u
 
 /* ------------------------------------------------------------------------ */
 /* Scratch memory: allocation_cache / allocation_group                      */

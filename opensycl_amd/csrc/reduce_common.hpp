@@ -137,9 +137,27 @@ struct params {
   int mode;
 };
 
-template <class T>
+// Device-scope (cross-XCD coherent) store / load of a group partial.
+template <class V>
+__device__ __forceinline__ void store_partial(V *dst, V v) {
+  static_assert(sizeof(V) == 4 || sizeof(V) == 8, "partials are 32 or 64 bits");
+  using U = std::conditional_t<sizeof(V) == 4, uint32_t, uint64_t>;
+  __hip_atomic_store(reinterpret_cast<U *>(dst), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class V>
+__device__ __forceinline__ V load_partial(V *src) {
+  using U = std::conditional_t<sizeof(V) == 4, uint32_t, uint64_t>;
+  return __builtin_bit_cast(V, __hip_atomic_load(reinterpret_cast<U *>(src), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <bool NT, class T>
 __device__ __forceinline__ u32x4 load16(const T *base, uint64_t chunk) {
-  return reinterpret_cast<const u32x4 *>(base)[chunk];
+  if constexpr (NT)
+    return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(base) + chunk);
+  else
+    return reinterpret_cast<const u32x4 *>(base)[chunk];
 }
 
 template <class T, class W, int VEC>
@@ -151,7 +169,7 @@ __device__ __forceinline__ void unpack(u32x4 q, W (&out)[VEC]) {
   for (int k = 0; k < VEC; ++k) out[k] = elem_traits<T>::to_work(__builtin_bit_cast(T, tmp[k]));
 }
 
-template <class T, class M, class Op, class OutT, int UNROLL, bool DOT>
+template <class T, class M, class Op, class OutT, int UNROLL, bool DOT, bool NT = false>
 __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   using W = typename elem_traits<T>::work;
   using V = typename M::value;
@@ -173,8 +191,8 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     u32x4 qb[UNROLL];
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-      qa[u] = load16(va, c + u * stride);
-      if constexpr (DOT) qb[u] = load16(vb, c + u * stride);
+      qa[u] = load16<NT>(va, c + u * stride);
+      if constexpr (DOT) qb[u] = load16<NT>(vb, c + u * stride);
     }
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
@@ -193,10 +211,10 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   }
   for (; c < p.nvec; c += stride) {
     W xa[VEC];
-    unpack<T, W, VEC>(load16(va, c), xa);
+    unpack<T, W, VEC>(load16<NT>(va, c), xa);
     if constexpr (DOT) {
       W xb[VEC];
-      unpack<T, W, VEC>(load16(vb, c), xb);
+      unpack<T, W, VEC>(load16<NT>(vb, c), xb);
 #pragma unroll
       for (int k = 0; k < VEC; ++k) acc[0] = M::insert2(acc[0], xa[k], xb[k]);
     } else {
@@ -222,15 +240,19 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   __shared__ int s_last;
   V bv = block_reduce<M>(v, s_waves);
   if (threadIdx.x == 0) {
-    p.partials[blockIdx.x] = bv;
-    __threadfence();
-    uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_ACQ_REL,
+    // The partial is written with a device-scope store and the ticket is
+    // taken only once that store has completed; the last group reads the
+    // partials with device-scope loads.  No release/acquire fence is needed,
+    // which on gfx950 would write back / invalidate this XCD's L2 once per
+    // work-group.
+    store_partial(p.partials + blockIdx.x, bv);
+    __builtin_amdgcn_s_waitcnt(0);
+    uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
     s_last = (prev == gridDim.x - 1);
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
 
   // Last work-group: fold the partials in a fixed order.
   const int nparts = gridDim.x;
@@ -239,7 +261,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     __shared__ double s_dwaves[max_waves];
     double d = -0.0;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-      V q = p.partials[i];
+      V q = load_partial(p.partials + i);
       d += static_cast<double>(q.s) + static_cast<double>(q.c);
     }
     d = block_reduce<double_plus<double>>(d, s_dwaves);
@@ -252,7 +274,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     }
   } else {
     V r = M::identity();
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) r = M::apply(r, p.partials[i]);
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) r = M::apply(r, load_partial(p.partials + i));
     __syncthreads();  // s_waves is reused
     r = block_reduce<M>(r, s_waves);
     if (threadIdx.x == 0) {

@@ -29,9 +29,10 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
   const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : 4;
-  if (unroll != 2 && unroll != 4 && unroll != 8)
+  if (unroll != 2 && unroll != 4 && unroll != 8 && (unroll != 16 || DOT))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
-                     std::string(name) + ": items_per_thread must be 2, 4 or 8");
+                     std::string(name) + ": items_per_thread must be 2, 4, 8 or 16 (16 only "
+                                         "for single-operand reductions)");
 
   // 16-byte aligned region; a dot product needs both operands equally
   // misaligned, otherwise everything goes through the scalar loop.
@@ -81,11 +82,20 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     return check_launch(name);
   };
 
-  // Only sums (the tuned hot path) get every unroll depth; other operators
-  // use 4 to bound the number of kernel instantiations.
+  // Only sums (the tuned hot path) get every unroll depth and the
+  // non-temporal variant; other operators use 4 to bound the number of
+  // kernel instantiations.
+  const bool nt = cfg && (cfg->flags & ACPP_VARIANT_NONTEMPORAL);
   if constexpr (Op::id == ops::op_plus) {
+    if (nt) {
+      if (unroll == 2) return run(reduce_kernel<T, M, Op, OutT, 2, DOT, true>);
+      if (unroll == 8) return run(reduce_kernel<T, M, Op, OutT, 8, DOT, true>);
+      if (unroll == 16) return run(reduce_kernel<T, M, Op, OutT, 16, DOT, true>);
+      return run(reduce_kernel<T, M, Op, OutT, 4, DOT, true>);
+    }
     if (unroll == 2) return run(reduce_kernel<T, M, Op, OutT, 2, DOT>);
     if (unroll == 8) return run(reduce_kernel<T, M, Op, OutT, 8, DOT>);
+    if (unroll == 16) return run(reduce_kernel<T, M, Op, OutT, 16, DOT>);
   }
   return run(reduce_kernel<T, M, Op, OutT, 4, DOT>);
 }

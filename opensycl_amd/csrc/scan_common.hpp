@@ -24,9 +24,10 @@
 //     values, each carrying the tag), written and read with relaxed
 //     device-scope atomics.  Words from earlier launches carry an older
 //     epoch and read as "invalid", so nothing has to be cleared;
-//   - the look-back is done by a whole wave: 64 predecessors are polled at
-//     once, the nearest "prefix" among them is found with a ballot, and the
-//     window is reduced with DPP;
+//   - the look-back is done by the whole work-group: blockDim.x
+//     predecessors are polled at once, the nearest "prefix" among them is
+//     found with ballots, and the window is reduced with DPP and one LDS
+//     exchange (a one-wave, 64-tile window is kept as a variant);
 //   - the last work-group to finish resets the tile ticket and the done
 //     counter and advances the epoch.
 #pragma once
@@ -125,6 +126,79 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch) {
   }
 }
 
+// Exclusive prefix of `tile` with a look-back window as wide as the whole
+// work-group: work-item t polls tile (tile - 1 - t), so one round trip to
+// memory covers blockDim.x predecessors.  The number of tiles a look-back
+// must cover grows with the number of tiles in flight, and every poll costs a
+// device-scope round trip, so the window width bounds the scan's throughput
+// at about (window / poll latency) tiles per unit time.  Must be called by
+// the whole work-group; the result is uniform.
+template <class Op, class W>
+__device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
+                            uint64_t *s_masks, W *s_red) {
+  using SIO = status_io<W>;
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int nw = B >> 6;
+  const int w = tid >> 6;
+  const int lane = tid & 63;
+  W excl = Op::identity();
+  int64_t base = static_cast<int64_t>(tile) - 1;
+  for (;;) {
+    const int64_t j = base - tid;  // work-item 0 = nearest predecessor
+    W val = Op::identity();
+    int p;
+    for (;;) {
+      uint32_t flag = st_prefix;  // positions before tile 0 act as an empty prefix
+      if (j >= 0) flag = SIO::read(status, static_cast<uint32_t>(j), epoch, val);
+      if (j < 0) val = Op::identity();
+      const uint64_t pm = wave::ballot(flag == st_prefix);
+      const uint64_t im = wave::ballot(flag == st_invalid);
+      if (lane == 0) {
+        s_masks[w] = pm;
+        s_masks[max_waves + w] = im;
+      }
+      __syncthreads();
+      p = B;
+      for (int ww = 0; ww < nw; ++ww) {
+        const uint64_t m = s_masks[ww];
+        if (m) {
+          p = ww * 64 + __builtin_ctzll(m);
+          break;
+        }
+      }
+      bool bad = false;
+      for (int ww = 0; ww < nw; ++ww) {
+        const int lo = ww * 64;
+        uint64_t need;
+        if (p >= lo + 63) need = ~0ull;
+        else if (p < lo) need = 0;
+        else need = (2ull << (p - lo)) - 1;
+        bad |= (s_masks[max_waves + ww] & need) != 0;
+      }
+      __syncthreads();  // the masks are rewritten by the next poll
+      if (!bad) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    // Fold the window [nearest .. p] and broadcast it.
+    W x = (tid <= p) ? val : Op::identity();
+    x = wave::reduce_full<Op>(x);
+    if (lane == 0) s_red[w] = x;
+    __syncthreads();
+    W window = Op::identity();
+    for (int ww = 0; ww < nw; ++ww) window = Op::apply(window, s_red[ww]);
+    __syncthreads();  // s_red is rewritten by the next window
+    excl = Op::apply(window, excl);
+    if (p < B) return excl;
+    base -= B;
+  }
+}
+
+// Launch variants (acpp_launch_config::flags).
+enum : uint32_t {
+  variant_wave_lookback = 1u,  // look back with one wave (64-tile window)
+};
+
 template <class T>
 struct params {
   using W = typename elem_traits<T>::work;
@@ -135,6 +209,7 @@ struct params {
   int has_init;
   int exclusive;
   int vec_io;
+  int wave_lookback;
   uint32_t num_tiles;
   uint64_t status_words;  // capacity of the status array, in 64-bit words
   sync_header *sync;
@@ -157,6 +232,8 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   __shared__ uint32_t s_epoch;
   __shared__ W s_wave[R][max_waves];
   __shared__ W s_excl;
+  __shared__ uint64_t s_masks[2 * max_waves];
+  __shared__ W s_red[max_waves];
 
   if (tid == 0) {
     s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED,
@@ -231,16 +308,22 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
       SIO::publish(p.status, 0, epoch, st_prefix, Op::apply(tile_prefix, agg));
   } else {
     if (tid == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
-    if (w == 0) {
-      const W excl = lookback<Op, W>(p.status, tile, epoch);
-      if (lane == 0) {
-        if (tile + 1 < p.num_tiles)
-          SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(excl, agg));
-        s_excl = excl;
+    if (p.wave_lookback) {
+      if (w == 0) {
+        const W excl = lookback<Op, W>(p.status, tile, epoch);
+        if (lane == 0) {
+          if (tile + 1 < p.num_tiles)
+            SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(excl, agg));
+          s_excl = excl;
+        }
       }
+      __syncthreads();
+      tile_prefix = s_excl;
+    } else {
+      tile_prefix = block_lookback<Op, W>(p.status, tile, epoch, s_masks, s_red);
+      if (tid == 0 && tile + 1 < p.num_tiles)
+        SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(tile_prefix, agg));
     }
-    __syncthreads();
-    tile_prefix = s_excl;
   }
 
 #pragma unroll
@@ -251,9 +334,16 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
       o[0] = static_cast<T>(P);
 #pragma unroll
       for (int k = 1; k < VEC; ++k) o[k] = static_cast<T>(Op::apply(P, v[r][k - 1]));
+      // d[0] is init itself (logical operators would turn op(init, identity)
+      // into 0/1).
+      if (r == 0 && tile == 0 && tid == 0) o[0] = static_cast<T>(p.init);
     } else {
 #pragma unroll
       for (int k = 0; k < VEC; ++k) o[k] = static_cast<T>(Op::apply(P, v[r][k]));
+      // Without init the first element is passed through untouched
+      // (std::inclusive_scan semantics).  op(identity, x) == x holds for
+      // every operator except the logical ones, which turn x into 0/1.
+      if (r == 0 && tile == 0 && tid == 0 && !p.has_init) o[0] = static_cast<T>(v[0][0]);
     }
     if (full) {
       u32x4 q;
@@ -268,9 +358,14 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
     }
   }
 
-  // The last work-group to finish makes the sync region reusable.
+  // The last work-group to finish makes the sync region reusable.  Work-item
+  // 0 made every status publication of this group; waiting for them to
+  // complete before counting keeps them ahead of the final reset without a
+  // release fence (which would write back the whole L2 of this XCD, once per
+  // tile).
   if (tid == 0) {
-    const uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_ACQ_REL,
+    __builtin_amdgcn_s_waitcnt(0);
+    const uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
     if (prev == p.num_tiles - 1) {
       __hip_atomic_store(&p.sync->tile_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

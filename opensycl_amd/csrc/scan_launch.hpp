@@ -58,6 +58,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   p.has_init = has_init ? 1 : 0;
   p.exclusive = exclusive ? 1 : 0;
   p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  p.wave_lookback = (cfg && (cfg->flags & variant_wave_lookback)) ? 1 : 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.status_words = (cap - sizeof(sync_header)) / sizeof(uint64_t);
   p.sync = static_cast<sync_header *>(region);

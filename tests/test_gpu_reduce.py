@@ -125,8 +125,10 @@ def test_reduction_semantics(gpu, q, scratch, dtype):
                         exact = math.fsum(x64) + (0.0 if to_id else float(start))
                         bound = depth * eps * (np.abs(x64).sum() + abs(float(start)))
                     else:
+                        # every one of the n multiplications contributes a
+                        # relative rounding error: |err| <= (n + 1) eps |prod|
                         exact = float(np.prod(x64)) * (1.0 if to_id else float(start))
-                        bound = 2 * depth * eps * abs(exact)
+                        bound = 1.01 * (n + 1) * eps * abs(exact)
                     assert abs(float(got) - exact) <= bound + 1e-300, (op, n, to_id)
                 else:
                     assert got == O.reduction(x, start, op, to_id), (op, n, to_id)
