@@ -226,6 +226,47 @@ acpp_status acpp_exclusive_scan_u64(acpp_allocation_group *scratch, acpp_stream_
 acpp_status acpp_exclusive_scan_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, float init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, double init, acpp_algorithm_op op, const acpp_launch_config *config);
 
+/* ------------------------------------------------------------------------ */
+/* Bulk atomics.  These replace a user kernel in which work-item i applies  */
+/* atomic_ref<T, relaxed, device>(target[slot(i)]).fetch_<op>(v_i)          */
+/* (include/hipSYCL/sycl/libkernel/atomic_ref.hpp:143-197, HIP lowering     */
+/* generic/hiplike/atomic_builtins.hpp:438-598).  The final contents of     */
+/* `target` equal those of that kernel (the operations commute); the        */
+/* per-element return values are not produced.                              */
+/*   strategy ACPP_ATOMICS_AUTO        library's choice                     */
+/*            ACPP_ATOMICS_PRIVATIZED  per-work-group copies in LDS /       */
+/*                                     registers, combined at the end       */
+/*            ACPP_ATOMICS_GLOBAL      one device-scope atomic per element  */
+/* ------------------------------------------------------------------------ */
+#define ACPP_ATOMICS_AUTO 0
+#define ACPP_ATOMICS_PRIVATIZED 1
+#define ACPP_ATOMICS_GLOBAL 2
+
+/* Histogram: bins[keys[i]] += 1 for every keys[i] < num_bins (other keys are
+   ignored); the counts are added to the existing bin values.  The
+   privatized strategy supports up to 65536 bins. */
+acpp_status acpp_histogram_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *keys, uint64_t n, uint32_t *bins, uint32_t num_bins, int strategy, const acpp_launch_config *config);
+
+/* Strided fetch ops: target[i % num_slots] = op(target[i % num_slots], values[i]). */
+acpp_status acpp_atomic_fetch_add_strided_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *values, uint64_t n, int32_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_min_strided_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *values, uint64_t n, int32_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_max_strided_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *values, uint64_t n, int32_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_add_strided_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *values, uint64_t n, uint32_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_min_strided_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *values, uint64_t n, uint32_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_max_strided_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *values, uint64_t n, uint32_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_add_strided_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *values, uint64_t n, int64_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_min_strided_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *values, uint64_t n, int64_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_max_strided_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *values, uint64_t n, int64_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_add_strided_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *values, uint64_t n, uint64_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_min_strided_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *values, uint64_t n, uint64_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_max_strided_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *values, uint64_t n, uint64_t *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_add_strided_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *values, uint64_t n, float *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_min_strided_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *values, uint64_t n, float *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_max_strided_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *values, uint64_t n, float *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_add_strided_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *values, uint64_t n, double *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_min_strided_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *values, uint64_t n, double *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+acpp_status acpp_atomic_fetch_max_strided_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *values, uint64_t n, double *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

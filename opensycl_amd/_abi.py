@@ -68,6 +68,11 @@ def _signatures() -> dict[str, tuple[list, object]]:
         S[f"acpp_reduce_compensated_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_float, cfg], c_int)
         S[f"acpp_transform_reduce_dot_compensated_{tag}"] = (
             [vp, vp, vp, c_uint64, vp, vp, c_float, cfg], c_int)
+    S["acpp_histogram_u32"] = ([vp, vp, vp, c_uint64, vp, c_uint32, c_int, cfg], c_int)
+    for op in ("add", "min", "max"):
+        for tag in ("i32", "u32", "i64", "u64", "f32", "f64"):
+            S[f"acpp_atomic_fetch_{op}_strided_{tag}"] = (
+                [vp, vp, vp, c_uint64, vp, c_uint32, c_int, cfg], c_int)
     return S
 
 

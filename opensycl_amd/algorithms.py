@@ -71,8 +71,8 @@ def _cfg(config):
         return ctypes.byref(config)
     if isinstance(config, dict):
         return ctypes.byref(LaunchConfig(**config))
-    wg, ipt, gpc = (tuple(config) + (0, 0, 0))[:3]
-    return ctypes.byref(LaunchConfig(wg, ipt, gpc, 0))
+    wg, ipt, gpc, flags = (tuple(config) + (0, 0, 0, 0))[:4]
+    return ctypes.byref(LaunchConfig(wg, ipt, gpc, flags))
 
 
 class Queue:
@@ -266,3 +266,36 @@ def exclusive_scan(q: Queue, scratch: AllocationGroup, x, d, init=0, op=PLUS,
     fn = getattr(_abi.load(), f"acpp_exclusive_scan_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _scalar(tag, init),
              _op(op), _cfg(config)))
+
+
+# ---------------------------------------------------------------------------
+# Bulk atomics (atomic_ref fetch_add / fetch_min / fetch_max in a loop)
+# ---------------------------------------------------------------------------
+ATOMICS_AUTO = 0
+ATOMICS_PRIVATIZED = 1
+ATOMICS_GLOBAL = 2
+
+
+def histogram(q: Queue, scratch: AllocationGroup, keys, bins, strategy: int = ATOMICS_AUTO,
+              config=None) -> None:
+    """bins[k] += number of keys equal to k (keys >= len(bins) are ignored)."""
+    _check_contig(keys, bins)
+    if _tag(keys.dtype) != "u32" or _tag(bins.dtype) != "u32":
+        raise TypeError("histogram takes uint32 keys and uint32 bins")
+    check(_abi.load().acpp_histogram_u32(scratch.handle, q.handle, _ptr(keys), keys.numel(),
+                                         _ptr(bins), bins.numel(), strategy, _cfg(config)))
+
+
+def atomic_fetch_strided(q: Queue, scratch: AllocationGroup, op: str, values, target,
+                         strategy: int = ATOMICS_AUTO, config=None) -> None:
+    """target[i % len(target)] = op(target[i % len(target)], values[i]) for op in
+    add / min / max."""
+    _check_contig(values, target)
+    if op not in ("add", "min", "max"):
+        raise ValueError("op must be add, min or max")
+    tag = _tag(values.dtype)
+    if values.dtype != target.dtype:
+        raise TypeError("values and target must have the same type")
+    fn = getattr(_abi.load(), f"acpp_atomic_fetch_{op}_strided_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(values), values.numel(), _ptr(target), target.numel(),
+             strategy, _cfg(config)))

@@ -100,6 +100,22 @@ def build(force: bool = True, jobs: int | None = None, verbose: bool = False) ->
     return LIB_PATH
 
 
+def build_test_kernels() -> str:
+    """Build tests/lib/libacpp_test_kernels.so: kernels that exercise the
+    device headers (group collectives, atomic_ref) for the GPU tests."""
+    src_dir = os.path.join(ROOT, "tests", "csrc")
+    out_dir = os.path.join(ROOT, "tests", "lib")
+    os.makedirs(out_dir, exist_ok=True)
+    srcs = sorted(os.path.join(src_dir, n) for n in os.listdir(src_dir) if n.endswith(".hip"))
+    out = os.path.join(out_dir, "libacpp_test_kernels.so")
+    cmd = [HIPCC, *COMMON_FLAGS, "-shared", "-o", out + ".tmp", *srcs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("test kernel build failed:\n" + r.stderr)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle() -> str:
     """Build the CPU oracle (test infrastructure) with its Makefile."""
     d = os.path.join(ROOT, "oracle")

@@ -46,11 +46,18 @@ acpp_status check_launch(const char *kernel_name);
 // (decoupled_lookback_scan.hpp:584-589) and any memset from the hot path.
 // ---------------------------------------------------------------------------
 struct sync_header {
-  uint32_t epoch;         // last epoch used by a completed launch (0 = none)
-  uint32_t tile_counter;  // dynamic tile ticket
-  uint32_t done_counter;  // number of work-groups that finished
-  uint32_t pad[61];       // header occupies 256 bytes
+  uint32_t epoch;           // unused by eager launches (epochs are kernel arguments)
+  uint32_t tile_counter;    // dynamic tile ticket, monotonic across launches
+  uint32_t done_counter;    // group ticket of the single-launch reductions
+  uint32_t watchdog_trips;  // look-backs that gave up waiting (never expected)
+  uint32_t pad[60];         // header occupies 256 bytes
 };
+
+// A look-back that polls this many times without progress gives up (and
+// counts a watchdog trip) instead of spinning forever.  Predecessor tiles
+// are always resident, so this only fires if the hardware stops making
+// progress; it keeps such a failure from hanging the device.
+constexpr uint32_t lookback_poll_limit = 1u << 22;
 static_assert(sizeof(sync_header) == 256, "sync header is 256 bytes");
 
 // Epochs live in 30 bits of a status word; 0 is never a live epoch.
@@ -61,14 +68,33 @@ constexpr uint32_t max_epoch = (1u << 30) - 1;
 // Internal part of the allocation group used by the kernels.
 struct acpp_allocation_group;
 namespace acpp_impl {
+
+// Host-side state of a sync region, kept with the region while it lives in
+// the allocation cache.  Eager launches pass the epoch and the ticket base as
+// kernel arguments: the host knows how many launches used the region and
+// how many tile tickets each one drew, so no kernel has to reset anything.
+struct region_state {
+  enum mode_t : int { fresh = 0, host_managed = 1, graph = 2 };
+  int mode = fresh;
+  uint32_t epoch = 0;        // epoch of the last launch (host_managed)
+  uint32_t ticket_base = 0;  // tile ticket counter value before the next launch
+};
+
 // Obtain ordinary scratch from the group.
 acpp_status group_obtain(acpp_allocation_group *g, size_t bytes, void **out);
 // Obtain a sync region of at least `bytes` (header included).  Fresh regions
 // are zeroed on `stream` before first use.  `capacity_bytes` receives the
-// region's real size.
+// region's real size, `state` its host-side state.
 acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
                               hipStream_t stream, void **out,
-                              size_t *capacity_bytes);
+                              size_t *capacity_bytes, region_state **state = nullptr);
+
+// Zero `bytes` of device memory with a kernel (also inside stream capture,
+// where memset nodes are avoided).
+acpp_status fill_zero(void *p, size_t bytes, hipStream_t stream);
+
+// True when `stream` is being captured into a graph.
+bool stream_is_capturing(hipStream_t stream);
 int group_device(const acpp_allocation_group *g);
 // Make the group's device current; returns its compute-unit count.
 acpp_status enter_device(acpp_allocation_group *g, int *compute_units);

@@ -8,6 +8,7 @@
 // pool of "sync regions" (zeroed once, kept reusable by the kernels), which
 // is what lets the scan and the reduction run as single launches.
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -73,6 +74,7 @@ struct allocation {
   size_t size = 0;
   int dev = 0;
   bool sync = false;
+  std::shared_ptr<region_state> state;  // sync regions only
 };
 }  // namespace
 
@@ -168,7 +170,7 @@ acpp_status group_obtain(acpp_allocation_group *g, size_t bytes, void **out) {
 
 acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
                               hipStream_t stream, void **out,
-                              size_t *capacity_bytes) {
+                              size_t *capacity_bytes, region_state **state) {
   if (g->parent->type != ACPP_ALLOCATION_DEVICE && g->parent->type != ACPP_ALLOCATION_SHARED)
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      "kernels need device-accessible scratch; use an "
@@ -179,15 +181,17 @@ acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
     acpp_status s = g->parent->allocate(bytes, g->dev, a);
     if (s != ACPP_SUCCESS) return s;
     a.sync = true;
-    hipError_t e = hipMemsetAsync(a.ptr, 0, a.size, stream);
-    if (e != hipSuccess) {
+    a.state = std::make_shared<region_state>();
+    s = fill_zero(a.ptr, a.size, stream);
+    if (s != ACPP_SUCCESS) {
       g->parent->deallocate(a);
-      return hip_error(e, "zeroing a new sync region");
+      return s;
     }
   }
   g->held.push_back(a);
   *out = a.ptr;
   *capacity_bytes = a.size;
+  if (state) *state = a.state.get();
   return ACPP_SUCCESS;
 }
 

@@ -28,8 +28,11 @@
 //     predecessors are polled at once, the nearest "prefix" among them is
 //     found with ballots, and the window is reduced with DPP and one LDS
 //     exchange (a one-wave, 64-tile window is kept as a variant);
-//   - the last work-group to finish resets the tile ticket and the done
-//     counter and advances the epoch.
+//   - nothing is reset at the end of a launch: the host tracks, per scratch
+//     region, the epoch of the last launch and how many tile tickets have
+//     been drawn, and passes both as kernel arguments (a per-tile "done"
+//     counter would add a second same-address atomic per tile);
+//   - a bounded-spin watchdog keeps a look-back from spinning forever.
 #pragma once
 
 #include "internal.hpp"
@@ -98,11 +101,13 @@ struct status_io<W, 8> {
 // Exclusive prefix of `tile`: op over all predecessors, found by a wave-wide
 // look-back.  Must be called by a whole wave; the result is uniform.
 template <class Op, class W>
-__device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch) {
+__device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
+                      uint32_t *watchdog) {
   using SIO = status_io<W>;
   const int lane = wave::lane_id();
   W excl = Op::identity();
   int64_t base = static_cast<int64_t>(tile) - 1;
+  uint32_t polls = 0;
   for (;;) {
     const int64_t j = base - lane;  // lane 0 = nearest predecessor
     W val = Op::identity();
@@ -111,6 +116,11 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch) {
       uint32_t flag = st_prefix;  // positions before tile 0 act as an empty prefix
       if (j >= 0) flag = SIO::read(status, static_cast<uint32_t>(j), epoch, val);
       if (j < 0) val = Op::identity();
+      if (++polls > lookback_poll_limit) {
+        if (lane == 0 && polls == lookback_poll_limit + 1)
+          __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flag = st_prefix;
+      }
       const uint64_t pm = wave::ballot(flag == st_prefix);
       p = pm ? __builtin_ctzll(pm) : 64;
       const uint64_t need = (p >= 63) ? ~0ull : ((2ull << p) - 1);
@@ -135,7 +145,7 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch) {
 // the whole work-group; the result is uniform.
 template <class Op, class W>
 __device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
-                            uint64_t *s_masks, W *s_red) {
+                            uint64_t *s_masks, W *s_red, uint32_t *watchdog) {
   using SIO = status_io<W>;
   const int tid = threadIdx.x;
   const int B = blockDim.x;
@@ -144,6 +154,7 @@ __device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoc
   const int lane = tid & 63;
   W excl = Op::identity();
   int64_t base = static_cast<int64_t>(tile) - 1;
+  uint32_t polls = 0;
   for (;;) {
     const int64_t j = base - tid;  // work-item 0 = nearest predecessor
     W val = Op::identity();
@@ -152,6 +163,11 @@ __device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoc
       uint32_t flag = st_prefix;  // positions before tile 0 act as an empty prefix
       if (j >= 0) flag = SIO::read(status, static_cast<uint32_t>(j), epoch, val);
       if (j < 0) val = Op::identity();
+      if (++polls > lookback_poll_limit) {
+        if (tid == 0 && polls == lookback_poll_limit + 1)
+          __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flag = st_prefix;
+      }
       const uint64_t pm = wave::ballot(flag == st_prefix);
       const uint64_t im = wave::ballot(flag == st_invalid);
       if (lane == 0) {
@@ -211,7 +227,8 @@ struct params {
   int vec_io;
   int wave_lookback;
   uint32_t num_tiles;
-  uint64_t status_words;  // capacity of the status array, in 64-bit words
+  uint32_t epoch;        // tag of this launch's status words (>= 1)
+  uint32_t ticket_base;  // tile ticket counter value when this launch starts
   sync_header *sync;
   uint64_t *status;
 };
@@ -229,20 +246,22 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   const int lane = tid & 63;
 
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_epoch;
   __shared__ W s_wave[R][max_waves];
   __shared__ W s_excl;
   __shared__ uint64_t s_masks[2 * max_waves];
   __shared__ W s_red[max_waves];
 
+  // Dynamic tile ids: tiles are handed out in the order work-groups start,
+  // so every predecessor of a tile is already running.  The counter is never
+  // reset; the host passes the value it had before this launch.
   if (tid == 0) {
     s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    s_epoch = __hip_atomic_load(&p.sync->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+                                    __HIP_MEMORY_SCOPE_AGENT) -
+             p.ticket_base;
   }
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint32_t epoch = s_epoch;
+  const uint32_t epoch = p.epoch;
 
   const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
@@ -310,7 +329,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
     if (tid == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
     if (p.wave_lookback) {
       if (w == 0) {
-        const W excl = lookback<Op, W>(p.status, tile, epoch);
+        const W excl = lookback<Op, W>(p.status, tile, epoch, &p.sync->watchdog_trips);
         if (lane == 0) {
           if (tile + 1 < p.num_tiles)
             SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(excl, agg));
@@ -320,7 +339,8 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
       __syncthreads();
       tile_prefix = s_excl;
     } else {
-      tile_prefix = block_lookback<Op, W>(p.status, tile, epoch, s_masks, s_red);
+      tile_prefix = block_lookback<Op, W>(p.status, tile, epoch, s_masks, s_red,
+                                         &p.sync->watchdog_trips);
       if (tid == 0 && tile + 1 < p.num_tiles)
         SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(tile_prefix, agg));
     }
@@ -355,29 +375,6 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
         const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
         if (idx < p.n) p.out[idx] = o[k];
       }
-    }
-  }
-
-  // The last work-group to finish makes the sync region reusable.  Work-item
-  // 0 made every status publication of this group; waiting for them to
-  // complete before counting keeps them ahead of the final reset without a
-  // release fence (which would write back the whole L2 of this XCD, once per
-  // tile).
-  if (tid == 0) {
-    __builtin_amdgcn_s_waitcnt(0);
-    const uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == p.num_tiles - 1) {
-      __hip_atomic_store(&p.sync->tile_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&p.sync->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t next = epoch;
-      if (epoch >= max_epoch) {
-        // Epoch space exhausted (once per 2^30 launches): clear every status
-        // word so that epoch 1 can be reused.
-        for (uint64_t i = 0; i < p.status_words; ++i) atomic_store_u64(p.status + i, 0);
-        next = 0;
-      }
-      __hip_atomic_store(&p.sync->epoch, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

@@ -47,8 +47,32 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   const uint64_t status_bytes = num_tiles * SIO::words * sizeof(uint64_t);
   void *region = nullptr;
   size_t cap = 0;
-  st = group_obtain_sync(g, sizeof(sync_header) + status_bytes, stream, &region, &cap);
+  region_state *rs = nullptr;
+  st = group_obtain_sync(g, sizeof(sync_header) + status_bytes, stream, &region, &cap, &rs);
   if (st != ACPP_SUCCESS) return st;
+
+  // Epoch and ticket base.  Eager launches continue the region's host-side
+  // count.  Under stream capture the graph may be replayed any number of
+  // times, so the captured work clears the region first (with a kernel) and
+  // always runs with epoch 1 and ticket base 0.
+  const bool capturing = stream_is_capturing(stream);
+  uint32_t epoch = 1, ticket_base = 0;
+  const size_t used = sizeof(sync_header) + status_bytes;
+  if (capturing) {
+    st = fill_zero(region, used, stream);
+    if (st != ACPP_SUCCESS) return st;
+    rs->mode = region_state::graph;
+  } else {
+    if (rs->mode == region_state::graph || rs->epoch >= max_epoch) {
+      st = fill_zero(region, cap, stream);
+      if (st != ACPP_SUCCESS) return st;
+      rs->epoch = 0;
+      rs->ticket_base = 0;
+    }
+    rs->mode = region_state::host_managed;
+    epoch = rs->epoch + 1;
+    ticket_base = rs->ticket_base;
+  }
 
   params<T> p;
   p.in = in;
@@ -60,13 +84,22 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
   p.wave_lookback = (cfg && (cfg->flags & variant_wave_lookback)) ? 1 : 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
-  p.status_words = (cap - sizeof(sync_header)) / sizeof(uint64_t);
+  p.epoch = epoch;
+  p.ticket_base = ticket_base;
   p.sync = static_cast<sync_header *>(region);
   p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
 
   auto run = [&](auto kernel) -> acpp_status {
     hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(num_tiles)), dim3(wg), 0, stream, p);
-    return check_launch(name);
+    acpp_status s = check_launch(name);
+    if (s == ACPP_SUCCESS && !capturing) {
+      rs->epoch = epoch;
+      rs->ticket_base = ticket_base + static_cast<uint32_t>(num_tiles);
+    } else if (s != ACPP_SUCCESS) {
+      // The launch may or may not have run: start the region over next time.
+      rs->mode = region_state::graph;
+    }
+    return s;
   };
   if constexpr (tunable) {
     if (rows == 2) return run(scan_kernel<T, Op, 2>);

@@ -1,0 +1,343 @@
+// Test kernels for the device headers (group collectives and atomic_ref).
+// Built into tests/lib/libacpp_test_kernels.so; the GPU tests call these
+// launchers through ctypes and compare with the oracle.  The kernels mirror
+// the reference's own tests:
+//   tests/sycl/group_functions/group_functions_{reduce,scan}.cpp
+//   tests/compiler/cbs/{reduce_for,babelstream_dot}.cpp
+//   tests/sycl/atomic.cpp, tests/pstl/std_atomic.cpp
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "acpp/device/atomic_ref.hpp"
+#include "acpp/device/group_functions.hpp"
+
+namespace {
+
+enum kind_t : int {
+  k_wg_reduce = 0,
+  k_wg_inclusive = 1,
+  k_wg_exclusive = 2,  // with init
+  k_sg_reduce = 3,
+  k_sg_inclusive = 4,
+  k_sg_exclusive = 5,  // with init
+  k_wg_reduce_init = 6,
+  k_wg_inclusive_init = 7,
+  k_wg_exclusive_identity = 8,  // exclusive scan without init (known identity)
+  // the same through the extern "C" builtin ABI
+  k_abi_wg_reduce = 10,
+  k_abi_wg_inclusive = 11,
+  k_abi_wg_exclusive = 12,
+  k_abi_sg_reduce = 13,
+  k_abi_sg_inclusive = 14,
+  k_abi_sg_exclusive = 15,
+};
+
+template <class T, class F>
+__device__ T run_kind(int kind, T x, T init, F op) {
+  acpp::group g = acpp::this_group();
+  acpp::sub_group sg = acpp::this_sub_group();
+  switch (kind) {
+    case k_wg_reduce: return acpp::reduce_over_group(g, x, op);
+    case k_wg_inclusive: return acpp::inclusive_scan_over_group(g, x, op);
+    case k_wg_exclusive: return acpp::exclusive_scan_over_group(g, x, init, op);
+    case k_sg_reduce: return acpp::reduce_over_group(sg, x, op);
+    case k_sg_inclusive: return acpp::inclusive_scan_over_group(sg, x, op);
+    case k_sg_exclusive: return acpp::exclusive_scan_over_group(sg, x, init, op);
+    case k_wg_reduce_init: return acpp::reduce_over_group(g, x, init, op);
+    case k_wg_inclusive_init: return acpp::inclusive_scan_over_group(g, x, init, op);
+    default: break;
+  }
+  if constexpr (acpp::known_identity<F, T>::value) {
+    if (kind == k_wg_exclusive_identity) return acpp::exclusive_scan_over_group(g, x, op);
+  }
+  return x;
+}
+
+template <class T>
+__device__ T run_op(int kind, int op, T x, T init) {
+  switch (op) {
+    case 0: return run_kind(kind, x, init, acpp::plus<T>{});
+    case 1: return run_kind(kind, x, init, acpp::multiplies<T>{});
+    case 2: return run_kind(kind, x, init, acpp::minimum<T>{});
+    case 3: return run_kind(kind, x, init, acpp::maximum<T>{});
+    default: break;
+  }
+  if constexpr (std::is_integral_v<T>) {
+    switch (op) {
+      case 4: return run_kind(kind, x, init, acpp::bit_and<T>{});
+      case 5: return run_kind(kind, x, init, acpp::bit_or<T>{});
+      case 6: return run_kind(kind, x, init, acpp::bit_xor<T>{});
+      case 7: return run_kind(kind, x, init, acpp::logical_and<T>{});
+      case 8: return run_kind(kind, x, init, acpp::logical_or<T>{});
+      default: break;
+    }
+  }
+  return x;
+}
+
+#define ABI_CALL(tag, T)                                                                    \
+  __device__ T abi_##tag(int kind, int op, T x, T init) {                                   \
+    switch (kind) {                                                                         \
+      case k_abi_wg_reduce: return __acpp_sscp_work_group_reduce_##tag(op, x);              \
+      case k_abi_wg_inclusive: return __acpp_sscp_work_group_inclusive_scan_##tag(op, x);   \
+      case k_abi_wg_exclusive: return __acpp_sscp_work_group_exclusive_scan_##tag(op, x, init); \
+      case k_abi_sg_reduce: return __acpp_sscp_sub_group_reduce_##tag(op, x);               \
+      case k_abi_sg_inclusive: return __acpp_sscp_sub_group_inclusive_scan_##tag(op, x);    \
+      default: return __acpp_sscp_sub_group_exclusive_scan_##tag(op, x, init);              \
+    }                                                                                       \
+  }
+ABI_CALL(i8, int8_t)
+ABI_CALL(i32, int32_t)
+ABI_CALL(i64, int64_t)
+ABI_CALL(u32, uint32_t)
+ABI_CALL(f32, float)
+ABI_CALL(f64, double)
+
+template <class T>
+__device__ T abi_dispatch(int kind, int op, T x, T init) {
+  if constexpr (std::is_same_v<T, int8_t>) return abi_i8(kind, op, x, init);
+  else if constexpr (std::is_same_v<T, int32_t>) return abi_i32(kind, op, x, init);
+  else if constexpr (std::is_same_v<T, int64_t>) return abi_i64(kind, op, x, init);
+  else if constexpr (std::is_same_v<T, uint32_t>) return abi_u32(kind, op, x, init);
+  else if constexpr (std::is_same_v<T, float>) return abi_f32(kind, op, x, init);
+  else return abi_f64(kind, op, x, init);
+}
+
+template <class T>
+__global__ void collective_kernel(int kind, int op, const T *in, T *out, T init) {
+  const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const T x = in[gid];
+  T r;
+  if (kind >= k_abi_wg_reduce)
+    r = abi_dispatch<T>(kind, op, x, init);
+  else
+    r = run_op<T>(kind, op, x, init);
+  out[gid] = r;
+}
+
+template <class T>
+__global__ void joint_kernel(int kind, int op, const T *in, T *out, uint64_t seg_len, T init) {
+  const T *first = in + static_cast<uint64_t>(blockIdx.x) * seg_len;
+  T *result = out + static_cast<uint64_t>(blockIdx.x) * seg_len;
+  acpp::group g = acpp::this_group();
+  // kinds: 0 joint_reduce(init) -> result[0], 1 joint_inclusive_scan,
+  // 2 joint_exclusive_scan(init), 3 joint_inclusive_scan(init),
+  // 4 joint_reduce without init (plus)
+  if (kind == 0) {
+    T r = T{};
+    switch (op) {
+      case 0: r = acpp::joint_reduce(g, first, first + seg_len, init, acpp::plus<T>{}); break;
+      case 2: r = acpp::joint_reduce(g, first, first + seg_len, init, acpp::minimum<T>{}); break;
+      default: r = acpp::joint_reduce(g, first, first + seg_len, init, acpp::maximum<T>{}); break;
+    }
+    if (g.leader()) result[0] = r;
+  } else if (kind == 1) {
+    acpp::joint_inclusive_scan(g, first, first + seg_len, result, acpp::plus<T>{});
+  } else if (kind == 2) {
+    acpp::joint_exclusive_scan(g, first, first + seg_len, result, init, acpp::plus<T>{});
+  } else if (kind == 3) {
+    acpp::joint_inclusive_scan(g, first, first + seg_len, result, acpp::plus<T>{}, init);
+  } else {
+    T r = acpp::joint_reduce(g, first, first + seg_len, acpp::plus<T>{});
+    if (g.leader()) result[0] = r;
+  }
+}
+
+// tests/compiler/cbs/babelstream_dot.cpp: 32 groups of 4 work-items, a
+// grid-stride dot product, then one partial per group.
+__global__ void babelstream_dot_kernel(const int32_t *a, const int32_t *b, int32_t *sums,
+                                       uint64_t n) {
+  const uint64_t gsize = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  int32_t acc = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += gsize)
+    acc += a[i] * b[i];
+  const int32_t r = acpp::reduce_over_group(acpp::this_group(), acc, acpp::plus<int32_t>{});
+  if (threadIdx.x == 0) sums[blockIdx.x] = r;
+}
+
+// tests/sycl/atomic.cpp:81-123: work-items 1..n-1 apply an atomic op with
+// operand data[i] to data[0].  rmw op numbering of the atomic builtin ABI:
+// and, or, xor, add, sub, min, max.
+template <class T, bool ABI>
+__global__ void atomic_fold_kernel(int op, T *data, uint64_t n) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i == 0 || i >= n) return;
+  const T x = data[i];
+  using acpp::memory_order;
+  using acpp::memory_scope;
+  if constexpr (ABI) {
+    const auto as = acpp::address_space::global_space;
+    const auto o = memory_order::relaxed;
+    const auto s = memory_scope::device;
+    if constexpr (std::is_same_v<T, int32_t>) {
+      switch (op) {
+        case 0: __acpp_sscp_atomic_fetch_and_i32(as, o, s, data, x); break;
+        case 1: __acpp_sscp_atomic_fetch_or_i32(as, o, s, data, x); break;
+        case 2: __acpp_sscp_atomic_fetch_xor_i32(as, o, s, data, x); break;
+        case 3: __acpp_sscp_atomic_fetch_add_i32(as, o, s, data, x); break;
+        case 4: __acpp_sscp_atomic_fetch_sub_i32(as, o, s, data, x); break;
+        case 5: __acpp_sscp_atomic_fetch_min_i32(as, o, s, data, x); break;
+        default: __acpp_sscp_atomic_fetch_max_i32(as, o, s, data, x); break;
+      }
+    } else if constexpr (std::is_same_v<T, float>) {
+      switch (op) {
+        case 3: __acpp_sscp_atomic_fetch_add_f32(as, o, s, data, x); break;
+        case 4: __acpp_sscp_atomic_fetch_sub_f32(as, o, s, data, x); break;
+        case 5: __acpp_sscp_atomic_fetch_min_f32(as, o, s, data, x); break;
+        default: __acpp_sscp_atomic_fetch_max_f32(as, o, s, data, x); break;
+      }
+    }
+  } else {
+    acpp::atomic_ref<T, memory_order::relaxed, memory_scope::device> r{data[0]};
+    if constexpr (std::is_integral_v<T>) {
+      switch (op) {
+        case 0: r.fetch_and(x); return;
+        case 1: r.fetch_or(x); return;
+        case 2: r.fetch_xor(x); return;
+        default: break;
+      }
+    }
+    switch (op) {
+      case 3: r.fetch_add(x); break;
+      case 4: r.fetch_sub(x); break;
+      case 5: r.fetch_min(x); break;
+      case 6: r.fetch_max(x); break;
+      default: break;
+    }
+  }
+}
+
+// tests/pstl/std_atomic.cpp:91-136: one work-item applies one operation and
+// records the previous value; also load/store/exchange/compare_exchange.
+template <class T>
+__global__ void atomic_single_kernel(int op, T *data, T operand, T *prev) {
+  acpp::atomic_ref<T, acpp::memory_order::relaxed, acpp::memory_scope::device> r{data[0]};
+  T p = T{};
+  switch (op) {
+    case 0: p = r.fetch_and(operand); break;
+    case 1: p = r.fetch_or(operand); break;
+    case 2: p = r.fetch_xor(operand); break;
+    case 3: p = r.fetch_add(operand); break;
+    case 4: p = r.fetch_sub(operand); break;
+    case 5: p = r.fetch_min(operand); break;
+    case 6: p = r.fetch_max(operand); break;
+    case 7: {  // load + store: x = x + 1
+      p = r.load();
+      r.store(p + 1);
+      break;
+    }
+    case 8: {  // exchange(1), then store(old + current)
+      p = r.exchange(1);
+      r.store(p + r.load());
+      break;
+    }
+    default: {  // compare_exchange: expected 0 fails, expected 42 succeeds
+      T e0 = 0;
+      bool ok0 = r.compare_exchange_strong(e0, operand);
+      T e1 = 42;
+      bool ok1 = r.compare_exchange_strong(e1, operand);
+      prev[1] = e0;
+      prev[2] = e1;
+      prev[3] = static_cast<T>(ok0 ? 1 : 0);
+      prev[4] = static_cast<T>(ok1 ? 1 : 0);
+      p = e1;
+      break;
+    }
+  }
+  prev[0] = p;
+}
+
+template <class T>
+int launch_collective(int kind, const void *in, void *out, uint64_t n, uint32_t local_size, int op,
+                      const void *init, hipStream_t s) {
+  if (local_size == 0 || local_size > 1024 || n % local_size) return 1;
+  T iv = init ? *static_cast<const T *>(init) : T{};
+  hipLaunchKernelGGL(collective_kernel<T>, dim3(static_cast<uint32_t>(n / local_size)), dim3(local_size),
+                     0, s, kind, op, static_cast<const T *>(in), static_cast<T *>(out), iv);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+template <class T>
+int launch_joint(int kind, const void *in, void *out, uint64_t n, uint64_t seg_len,
+                 uint32_t local_size, int op, const void *init, hipStream_t s) {
+  if (seg_len == 0 || n % seg_len || local_size == 0 || local_size > 1024) return 1;
+  T iv = init ? *static_cast<const T *>(init) : T{};
+  hipLaunchKernelGGL(joint_kernel<T>, dim3(static_cast<uint32_t>(n / seg_len)), dim3(local_size), 0, s,
+                     kind, op, static_cast<const T *>(in), static_cast<T *>(out), seg_len, iv);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+// tag: 0 i8, 1 i32, 2 i64, 3 u32, 4 f32, 5 f64
+int tk_group_collective(int tag, int kind, const void *in, void *out, uint64_t n,
+                        uint32_t local_size, int op, const void *init, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (tag) {
+    case 0: return launch_collective<int8_t>(kind, in, out, n, local_size, op, init, s);
+    case 1: return launch_collective<int32_t>(kind, in, out, n, local_size, op, init, s);
+    case 2: return launch_collective<int64_t>(kind, in, out, n, local_size, op, init, s);
+    case 3: return launch_collective<uint32_t>(kind, in, out, n, local_size, op, init, s);
+    case 4: return launch_collective<float>(kind, in, out, n, local_size, op, init, s);
+    case 5: return launch_collective<double>(kind, in, out, n, local_size, op, init, s);
+    default: return 1;
+  }
+}
+
+int tk_joint(int tag, int kind, const void *in, void *out, uint64_t n, uint64_t seg_len,
+             uint32_t local_size, int op, const void *init, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (tag) {
+    case 1: return launch_joint<int32_t>(kind, in, out, n, seg_len, local_size, op, init, s);
+    case 2: return launch_joint<int64_t>(kind, in, out, n, seg_len, local_size, op, init, s);
+    case 4: return launch_joint<float>(kind, in, out, n, seg_len, local_size, op, init, s);
+    default: return 1;
+  }
+}
+
+int tk_babelstream_dot(const int32_t *a, const int32_t *b, int32_t *sums, uint64_t n,
+                       uint32_t num_groups, uint32_t local_size, void *stream) {
+  if (local_size == 0 || local_size > 1024 || num_groups == 0) return 1;
+  hipLaunchKernelGGL(babelstream_dot_kernel, dim3(num_groups), dim3(local_size), 0,
+                     static_cast<hipStream_t>(stream), a, b, sums, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+// tag: 1 i32, 2 i64, 3 u32, 4 f32, 5 f64, 6 u64; abi: use the extern "C"
+// builtins (i32 and f32 only)
+int tk_atomic_fold(int tag, int op, void *data, uint64_t n, int abi, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid(static_cast<uint32_t>((n + 255) / 256)), block(256);
+  if (abi) {
+    if (tag == 1) hipLaunchKernelGGL((atomic_fold_kernel<int32_t, true>), grid, block, 0, s, op, static_cast<int32_t *>(data), n);
+    else if (tag == 4) hipLaunchKernelGGL((atomic_fold_kernel<float, true>), grid, block, 0, s, op, static_cast<float *>(data), n);
+    else return 1;
+  } else {
+    switch (tag) {
+      case 1: hipLaunchKernelGGL((atomic_fold_kernel<int, false>), grid, block, 0, s, op, static_cast<int *>(data), n); break;
+      case 2: hipLaunchKernelGGL((atomic_fold_kernel<long long, false>), grid, block, 0, s, op, static_cast<long long *>(data), n); break;
+      case 3: hipLaunchKernelGGL((atomic_fold_kernel<unsigned int, false>), grid, block, 0, s, op, static_cast<unsigned int *>(data), n); break;
+      case 4: hipLaunchKernelGGL((atomic_fold_kernel<float, false>), grid, block, 0, s, op, static_cast<float *>(data), n); break;
+      case 5: hipLaunchKernelGGL((atomic_fold_kernel<double, false>), grid, block, 0, s, op, static_cast<double *>(data), n); break;
+      case 6: hipLaunchKernelGGL((atomic_fold_kernel<unsigned long long, false>), grid, block, 0, s, op, static_cast<unsigned long long *>(data), n); break;
+      default: return 1;
+    }
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int tk_atomic_single(int tag, int op, void *data, const void *operand, void *prev, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (tag) {
+    case 1: hipLaunchKernelGGL(atomic_single_kernel<int>, dim3(1), dim3(1), 0, s, op, static_cast<int *>(data), *static_cast<const int *>(operand), static_cast<int *>(prev)); break;
+    case 2: hipLaunchKernelGGL(atomic_single_kernel<long long>, dim3(1), dim3(1), 0, s, op, static_cast<long long *>(data), *static_cast<const long long *>(operand), static_cast<long long *>(prev)); break;
+    case 3: hipLaunchKernelGGL(atomic_single_kernel<unsigned int>, dim3(1), dim3(1), 0, s, op, static_cast<unsigned int *>(data), *static_cast<const unsigned int *>(operand), static_cast<unsigned int *>(prev)); break;
+    case 6: hipLaunchKernelGGL(atomic_single_kernel<unsigned long long>, dim3(1), dim3(1), 0, s, op, static_cast<unsigned long long *>(data), *static_cast<const unsigned long long *>(operand), static_cast<unsigned long long *>(prev)); break;
+    default: return 1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+}  // extern "C"

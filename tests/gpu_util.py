@@ -42,3 +42,14 @@ def op_input(dtype, op, n, seed):
         x = np.bitwise_or(x, ~np.array(1 << (np.random.default_rng(seed).integers(0, 7)),
                                        dtype=dtype))
     return x
+
+
+def zipf_keys(n, num_bins, s=1.1, seed=0):
+    """Bounded Zipf(s) keys on {0..num_bins-1} by inverse CDF
+    (p_k proportional to (k+1)^-s)."""
+    rng = np.random.default_rng(seed)
+    p = np.arange(1, num_bins + 1, dtype=np.float64) ** -s
+    cdf = np.cumsum(p)
+    cdf /= cdf[-1]
+    u = rng.random(n)
+    return np.minimum(np.searchsorted(cdf, u, side="right"), num_bins - 1).astype(np.uint32)
