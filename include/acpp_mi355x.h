@@ -267,6 +267,46 @@ acpp_status acpp_atomic_fetch_add_strided_f64(acpp_allocation_group *scratch, ac
 acpp_status acpp_atomic_fetch_min_strided_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *values, uint64_t n, double *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
 acpp_status acpp_atomic_fetch_max_strided_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *values, uint64_t n, double *target, uint32_t num_slots, int strategy, const acpp_launch_config *config);
 
+/* ------------------------------------------------------------------------ */
+/* Segmented scans and reductions: independent scans / reductions over      */
+/* segments [offsets[s], offsets[s+1]) of one array, in one launch.  No     */
+/* counterpart in the reference's algorithms library; the semantics are one */
+/* inclusive_scan (numeric.hpp:253-279, no init) or reduce                  */
+/* (numeric.hpp:226-249) per segment, i.e. one joint_inclusive_scan /       */
+/* joint_reduce (generic/hiplike/group_functions.hpp:214-287, 361-432) per  */
+/* segment.                                                                 */
+/*   offsets: num_segments + 1 device values, non-decreasing, offsets[0] = 0,*/
+/*            offsets[num_segments] = n.  Malformed offsets give wrong      */
+/*            results but no out-of-bounds access.                          */
+/*   reduce:  out[s] = op(init, x[offsets[s]] op ... op x[offsets[s+1]-1]); */
+/*            empty segments get init.                                      */
+/*   op:      plus, multiply, min or max.                                   */
+/* ------------------------------------------------------------------------ */
+acpp_status acpp_segmented_inclusive_scan_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_segmented_inclusive_scan_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_segmented_inclusive_scan_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_segmented_inclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_segmented_reduce_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, int32_t *out, int32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_segmented_reduce_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, int64_t *out, int64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_segmented_reduce_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, float *out, float init, acpp_algorithm_op op, const acpp_launch_config *config);
+acpp_status acpp_segmented_reduce_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, double *out, double init, acpp_algorithm_op op, const acpp_launch_config *config);
+
+/* hipGraph plans: the same work captured once (setup kernel + persistent
+   kernel, scratch from `scratch`) and replayed with one graph launch.  The
+   plan references `scratch`, the input, output and offsets buffers: keep
+   them alive (and the group unreleased) while the plan is used. */
+typedef struct acpp_segmented_plan acpp_segmented_plan;
+acpp_status acpp_segmented_plan_inclusive_scan_i32(acpp_allocation_group *scratch, const int32_t *first, uint64_t n, int32_t *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_inclusive_scan_i64(acpp_allocation_group *scratch, const int64_t *first, uint64_t n, int64_t *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_inclusive_scan_f32(acpp_allocation_group *scratch, const float *first, uint64_t n, float *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_inclusive_scan_f64(acpp_allocation_group *scratch, const double *first, uint64_t n, double *d_first, const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_reduce_i32(acpp_allocation_group *scratch, const int32_t *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, int32_t *out, int32_t init, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_reduce_i64(acpp_allocation_group *scratch, const int64_t *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, int64_t *out, int64_t init, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_reduce_f32(acpp_allocation_group *scratch, const float *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, float *out, float init, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_reduce_f64(acpp_allocation_group *scratch, const double *first, uint64_t n, const uint64_t *offsets, uint64_t num_segments, double *out, double init, acpp_algorithm_op op, const acpp_launch_config *config, acpp_segmented_plan **plan);
+acpp_status acpp_segmented_plan_launch(acpp_segmented_plan *plan, acpp_stream_t stream);
+acpp_status acpp_segmented_plan_destroy(acpp_segmented_plan *plan);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

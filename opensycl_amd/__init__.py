@@ -6,7 +6,7 @@ reached through the C-ABI in ``include/acpp_mi355x.h``.
 from . import _abi  # noqa: F401
 from .algorithms import (ALLOCATION_DEVICE, ALLOCATION_HOST, ALLOCATION_SHARED,  # noqa: F401
                          ATOMICS_AUTO, ATOMICS_GLOBAL, ATOMICS_PRIVATIZED, atomic_fetch_strided,
-                         histogram,
+                         histogram, SegmentedPlan, segmented_inclusive_scan, segmented_reduce,
                          BIT_AND, BIT_OR, BIT_XOR, LOGICAL_AND, LOGICAL_OR, MAX, MIN, MULTIPLY,
                          PLUS, AllocationCache, AllocationGroup, Queue, compute_units,
                          exclusive_scan, inclusive_scan, reduce, reduce_compensated, reduction,

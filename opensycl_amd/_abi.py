@@ -68,6 +68,18 @@ def _signatures() -> dict[str, tuple[list, object]]:
         S[f"acpp_reduce_compensated_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_float, cfg], c_int)
         S[f"acpp_transform_reduce_dot_compensated_{tag}"] = (
             [vp, vp, vp, c_uint64, vp, vp, c_float, cfg], c_int)
+    for tag in ("i32", "i64", "f32", "f64"):
+        ct = SCALAR_TAGS[tag]
+        S[f"acpp_segmented_inclusive_scan_{tag}"] = (
+            [vp, vp, vp, c_uint64, vp, vp, c_uint64, c_int32, cfg], c_int)
+        S[f"acpp_segmented_reduce_{tag}"] = (
+            [vp, vp, vp, c_uint64, vp, c_uint64, vp, ct, c_int32, cfg], c_int)
+        S[f"acpp_segmented_plan_inclusive_scan_{tag}"] = (
+            [vp, vp, c_uint64, vp, vp, c_uint64, c_int32, cfg, POINTER(vp)], c_int)
+        S[f"acpp_segmented_plan_reduce_{tag}"] = (
+            [vp, vp, c_uint64, vp, c_uint64, vp, ct, c_int32, cfg, POINTER(vp)], c_int)
+    S["acpp_segmented_plan_launch"] = ([vp, vp], c_int)
+    S["acpp_segmented_plan_destroy"] = ([vp], c_int)
     S["acpp_histogram_u32"] = ([vp, vp, vp, c_uint64, vp, c_uint32, c_int, cfg], c_int)
     for op in ("add", "min", "max"):
         for tag in ("i32", "u32", "i64", "u64", "f32", "f64"):

@@ -299,3 +299,76 @@ def atomic_fetch_strided(q: Queue, scratch: AllocationGroup, op: str, values, ta
     fn = getattr(_abi.load(), f"acpp_atomic_fetch_{op}_strided_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(values), values.numel(), _ptr(target), target.numel(),
              strategy, _cfg(config)))
+
+
+# ---------------------------------------------------------------------------
+# Segmented scans and reductions
+# ---------------------------------------------------------------------------
+def _seg_check(x, offsets):
+    if _tag(offsets.dtype) != "u64":
+        raise TypeError("offsets must be uint64")
+    _check_contig(x, offsets)
+
+
+def segmented_inclusive_scan(q: Queue, scratch: AllocationGroup, x, d, offsets, op=PLUS,
+                             config=None) -> None:
+    """d[i] = inclusive scan of x within the segment [offsets[s], offsets[s+1])."""
+    _seg_check(x, offsets)
+    _check_contig(d)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_segmented_inclusive_scan_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _ptr(offsets),
+             offsets.numel() - 1, _op(op), _cfg(config)))
+
+
+def segmented_reduce(q: Queue, scratch: AllocationGroup, x, offsets, out, init=0, op=PLUS,
+                     config=None) -> None:
+    """out[s] = op(init, fold of x over [offsets[s], offsets[s+1]))."""
+    _seg_check(x, offsets)
+    _check_contig(out)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_segmented_reduce_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(offsets), offsets.numel() - 1,
+             _ptr(out), _scalar(tag, init), _op(op), _cfg(config)))
+
+
+class SegmentedPlan:
+    """A segmented scan or reduction captured into a hipGraph once and
+    replayed with ``launch``."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def inclusive_scan(scratch: AllocationGroup, x, d, offsets, op=PLUS, config=None):
+        _seg_check(x, offsets)
+        tag = _tag(x.dtype)
+        h = ctypes.c_void_p()
+        fn = getattr(_abi.load(), f"acpp_segmented_plan_inclusive_scan_{tag}")
+        check(fn(scratch.handle, _ptr(x), x.numel(), _ptr(d), _ptr(offsets), offsets.numel() - 1,
+                 _op(op), _cfg(config), ctypes.byref(h)))
+        return SegmentedPlan(h)
+
+    @staticmethod
+    def reduce(scratch: AllocationGroup, x, offsets, out, init=0, op=PLUS, config=None):
+        _seg_check(x, offsets)
+        tag = _tag(x.dtype)
+        h = ctypes.c_void_p()
+        fn = getattr(_abi.load(), f"acpp_segmented_plan_reduce_{tag}")
+        check(fn(scratch.handle, _ptr(x), x.numel(), _ptr(offsets), offsets.numel() - 1,
+                 _ptr(out), _scalar(tag, init), _op(op), _cfg(config), ctypes.byref(h)))
+        return SegmentedPlan(h)
+
+    def launch(self, q: Queue) -> None:
+        check(_abi.load().acpp_segmented_plan_launch(self._h, q.handle))
+
+    def close(self) -> None:
+        if self._h:
+            check(_abi.load().acpp_segmented_plan_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

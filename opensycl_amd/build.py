@@ -116,6 +116,20 @@ def build_test_kernels() -> str:
     return out
 
 
+def build_bench_helpers() -> str:
+    """Build opensycl_amd/lib/libacpp_bench.so: native per-segment call loops
+    used as the baseline of the segmented benchmarks."""
+    src = os.path.join(PKG_DIR, "bench", "per_segment.cpp")
+    out = os.path.join(LIB_DIR, "libacpp_bench.so")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INCLUDE}", "-o", out + ".tmp", src,
+           f"-L{LIB_DIR}", "-lacpp_mi355x", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("bench helper build failed:\n" + r.stderr)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle() -> str:
     """Build the CPU oracle (test infrastructure) with its Makefile."""
     d = os.path.join(ROOT, "oracle")

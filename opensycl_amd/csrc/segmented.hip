@@ -1,0 +1,751 @@
+// Segmented scans and reductions: thousands of independent scans /
+// reductions over [offsets[s], offsets[s+1]) in one launch.
+//
+// The reference has no segmented form; the nearest equivalents are one
+// joint_* group algorithm call per segment
+// (include/hipSYCL/sycl/libkernel/generic/hiplike/group_functions.hpp:214-287,
+// 361-432) or one algorithms::inclusive_scan / reduce call per segment
+// (include/hipSYCL/algorithms/numeric.hpp:226-279), each a separate launch.
+//
+// MI355X design:
+//   - the concatenated elements are cut into equal tiles, independent of the
+//     segment lengths, so log-normal segment lengths cost no load imbalance;
+//   - a setup kernel builds the device work queue: for every tile the first
+//     segment it touches (each non-empty segment writes the tiles whose first
+//     element it contains), and writes init into empty segments' results;
+//   - persistent work-groups (one wave of resident groups) draw tiles from
+//     a ticket counter in order; every tile marks its segment starts in an
+//     LDS bitmap and runs a segmented scan over (head flag, value) pairs:
+//     (f1,v1) . (f2,v2) = (f1|f2, f2 ? v2 : v1 op v2), in registers, across
+//     each wave with DPP and across waves through LDS;
+//   - tiles chain through a decoupled look-back on the pair operator.  The
+//     operator is not commutative, so the window is folded in tile order,
+//     and a look-back stops at the nearest predecessor that contains a
+//     segment start (its aggregate already discards everything before it),
+//     so it is short whenever segments are shorter than a tile;
+//   - reductions write op(init, value) at the last element of each segment
+//     only; scans write every element;
+//   - the setup and main kernels can be captured into a hipGraph once and
+//     replayed (acpp_segmented_plan_*).
+#include <memory>
+
+#include "launch_util.hpp"
+
+namespace acpp_impl {
+namespace seg_detail {
+
+constexpr int max_waves = 16;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+enum : uint32_t { st_invalid = 0, st_aggregate = 1, st_prefix = 2 };
+
+// Tag layout: epoch (29 bits) << 3 | head flag << 2 | status (2 bits).
+constexpr uint32_t seg_max_epoch = (1u << 29) - 1;
+
+__device__ __forceinline__ uint64_t ld64(const uint64_t *p) {
+  return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st64(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class W, int Bytes = sizeof(W)>
+struct seg_status;
+
+template <class W>
+struct seg_status<W, 4> {
+  static constexpr int words = 1;
+  __device__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t status,
+                                 uint32_t f, W v) {
+    const uint64_t tag = (static_cast<uint64_t>(epoch) << 3) | (f << 2) | status;
+    st64(st + tile, (tag << 32) | __builtin_bit_cast(uint32_t, v));
+  }
+  __device__ static uint32_t read(const uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t &f,
+                                  W &v) {
+    const uint64_t w = ld64(st + tile);
+    const uint32_t tag = static_cast<uint32_t>(w >> 32);
+    if ((tag >> 3) != epoch) return st_invalid;
+    f = (tag >> 2) & 1u;
+    v = __builtin_bit_cast(W, static_cast<uint32_t>(w));
+    return tag & 3u;
+  }
+};
+
+template <class W>
+struct seg_status<W, 8> {
+  static constexpr int words = 2;
+  __device__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t status,
+                                 uint32_t f, W v) {
+    const uint64_t tag = (static_cast<uint64_t>(epoch) << 3) | (f << 2) | status;
+    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+    st64(st + 2 * static_cast<uint64_t>(tile), (tag << 32) | (bits & 0xffffffffu));
+    st64(st + 2 * static_cast<uint64_t>(tile) + 1, (tag << 32) | (bits >> 32));
+  }
+  __device__ static uint32_t read(const uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t &f,
+                                  W &v) {
+    const uint64_t w0 = ld64(st + 2 * static_cast<uint64_t>(tile));
+    const uint64_t w1 = ld64(st + 2 * static_cast<uint64_t>(tile) + 1);
+    const uint32_t t0 = static_cast<uint32_t>(w0 >> 32);
+    const uint32_t t1 = static_cast<uint32_t>(w1 >> 32);
+    if (t0 != t1 || (t0 >> 3) != epoch) return st_invalid;
+    f = (t0 >> 2) & 1u;
+    v = __builtin_bit_cast(W, (w1 << 32) | (w0 & 0xffffffffu));
+    return t0 & 3u;
+  }
+};
+
+// (f, v) pair operator, earlier . later
+template <class Op, class W>
+__device__ __forceinline__ void combine(uint32_t fa, W va, uint32_t &fb, W &vb) {
+  vb = fb ? vb : Op::apply(va, vb);
+  fb |= fa;
+}
+
+// Inclusive scan of pairs across the wave (lane order = element order).
+template <class Op, class W>
+__device__ __forceinline__ void wave_pair_scan(uint32_t &f, W &v) {
+  const W id = Op::identity();
+#define ACPP_SEG_STEP(CTRL, RM)                                  \
+  {                                                              \
+    const uint32_t fo = wave::dpp<CTRL, RM, 0xf>(0u, f);         \
+    const W vo = wave::dpp<CTRL, RM, 0xf>(id, v);                \
+    combine<Op>(fo, vo, f, v);                                   \
+  }
+  ACPP_SEG_STEP(wave::row_shr1, 0xf)
+  ACPP_SEG_STEP(wave::row_shr2, 0xf)
+  ACPP_SEG_STEP(wave::row_shr4, 0xf)
+  ACPP_SEG_STEP(wave::row_shr8, 0xf)
+  ACPP_SEG_STEP(wave::row_bcast15, 0xa)
+  ACPP_SEG_STEP(wave::row_bcast31, 0xc)
+#undef ACPP_SEG_STEP
+}
+
+template <class T>
+struct params {
+  using W = typename elem_traits<T>::work;
+  const T *in;
+  T *out;
+  const uint64_t *offsets;  // num_segments + 1 entries
+  const uint32_t *tile_seg;  // first segment of every tile (num_tiles + 1 entries)
+  uint64_t num_segments;
+  uint64_t total;
+  W init;
+  int reduce;  // 0: inclusive scan per segment, 1: reduce per segment
+  int vec_io;
+  uint32_t num_tiles;
+  uint32_t epoch;
+  uint32_t ticket_base;
+  sync_header *sync;
+  uint64_t *status;
+};
+
+__device__ __forceinline__ uint64_t clamp_off(const uint64_t *offsets, uint64_t s, uint64_t total) {
+  const uint64_t o = offsets[s];
+  return o < total ? o : total;
+}
+
+// Setup: tile_seg[t] = the non-empty segment containing element t*tile_elems
+// (num_segments for t == num_tiles), and init into empty segments' results.
+template <class T>
+__global__ void setup_kernel(params<T> p, uint32_t *tile_seg_out, uint64_t tile_elems) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < p.num_segments;
+       s += stride) {
+    const uint64_t b = clamp_off(p.offsets, s, p.total);
+    const uint64_t e = clamp_off(p.offsets, s + 1, p.total);
+    if (e <= b) {
+      if (p.reduce) p.out[s] = static_cast<T>(p.init);
+      continue;
+    }
+    // tiles whose first element lies in [b, e)
+    const uint64_t t0 = (b + tile_elems - 1) / tile_elems;
+    const uint64_t t1 = (e + tile_elems - 1) / tile_elems;
+    for (uint64_t t = t0; t < t1; ++t) tile_seg_out[t] = static_cast<uint32_t>(s);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) tile_seg_out[p.num_tiles] = static_cast<uint32_t>(p.num_segments);
+}
+
+// Largest s in [lo, hi] with offsets[s] <= e (offsets[lo] <= e holds).
+__device__ __forceinline__ uint64_t find_segment(const uint64_t *offsets, uint64_t lo, uint64_t hi,
+                                                 uint64_t e, uint64_t total) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) / 2;
+    if (clamp_off(offsets, mid, total) <= e)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T, class Op, int R>
+__global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
+  using W = typename elem_traits<T>::work;
+  using SIO = seg_status<W>;
+  constexpr int VEC = 16 / sizeof(T);
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int nw = B >> 6;
+  const int w = tid >> 6;
+  const int lane = tid & 63;
+  const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
+  const W id = Op::identity();
+
+  extern __shared__ uint32_t s_heads[];  // tile_elems + 1 bits
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wf[R][max_waves];
+  __shared__ W s_wv[R][max_waves];
+  __shared__ uint32_t s_lbf[max_waves];
+  __shared__ W s_lbv[max_waves];
+  __shared__ int s_q[max_waves];
+  __shared__ uint64_t s_invalid[max_waves];
+  const uint32_t head_words = static_cast<uint32_t>((tile_elems + 1 + 31) / 32);
+
+  for (;;) {
+    __syncthreads();  // LDS of the previous tile is free again
+    if (tid == 0)
+      s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) -
+               p.ticket_base;
+    for (uint32_t i = tid; i < head_words; i += B) s_heads[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= p.num_tiles) return;  // queue drained
+
+    const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+    const uint64_t tile_end = (base + tile_elems < p.total) ? base + tile_elems : p.total;
+    // Segments touching the tile: [s_lo, s_hi].  Indices are clamped so that
+    // malformed offsets can give wrong results but never out-of-bounds
+    // accesses.
+    uint64_t s_lo = p.tile_seg[tile];
+    uint64_t s_hi = p.tile_seg[tile + 1];
+    if (s_lo >= p.num_segments) s_lo = p.num_segments - 1;
+    if (s_hi >= p.num_segments) s_hi = p.num_segments - 1;
+    if (s_hi < s_lo) s_hi = s_lo;
+    // Mark segment starts in [base, tile_end] (bit tile_elems: a segment
+    // starts exactly at the next tile).
+    for (uint64_t s = s_lo + tid; s <= s_hi; s += B) {
+      const uint64_t o = clamp_off(p.offsets, s, p.total);
+      if (o >= base && o <= tile_end && o < p.total) {
+        const uint64_t pos = o - base;
+        atomicOr(&s_heads[pos >> 5], 1u << (pos & 31));
+      }
+    }
+    __syncthreads();
+
+    auto head_at = [&](uint64_t pos) -> uint32_t {
+      return (s_heads[pos >> 5] >> (pos & 31)) & 1u;
+    };
+
+    // Load the tile.
+    const bool full = p.vec_io && (base + tile_elems <= p.total);
+    W v[R][VEC];
+    // fm[r] bit k: element k of row r starts a segment.
+    uint32_t fm[R];
+    if (full) {
+      const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
+      u32x4 q[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T tmp[VEC];
+        __builtin_memcpy(tmp, &q[r], 16);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+          const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
+          v[r][k] = (idx < p.total) ? elem_traits<T>::to_work(p.in[idx]) : id;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // VEC consecutive bits of the bitmap (VEC divides 32, so they never
+      // straddle a word).
+      const uint64_t pos = static_cast<uint64_t>(r * B + tid) * VEC;
+      fm[r] = (s_heads[pos >> 5] >> (pos & 31)) & ((1u << VEC) - 1u);
+    }
+
+    // Work-item-local segmented scan of each row vector: an element that
+    // starts a segment keeps its value, the others add the running value.
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 1; k < VEC; ++k)
+        v[r][k] = ((fm[r] >> k) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
+
+    // Wave scans of the row totals.
+    uint32_t xf[R];
+    W xv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      uint32_t tf = fm[r] != 0u;
+      W tv = v[r][VEC - 1];
+      wave_pair_scan<Op>(tf, tv);
+      if (lane == 63) {
+        s_wf[r][w] = tf;
+        s_wv[r][w] = tv;
+      }
+      xf[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(0u, tf);
+      xv[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(id, tv);
+    }
+    __syncthreads();
+    // Pairs before (row r, wave w) inside the tile, in element order.
+    uint32_t bf[R];
+    W bv[R];
+    uint32_t cf = 0;
+    W cv = id;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      bf[r] = 0;
+      bv[r] = id;
+      for (int ww = 0; ww < nw; ++ww) {
+        if (ww == w) {
+          bf[r] = cf;
+          bv[r] = cv;
+        }
+        uint32_t yf = s_wf[r][ww];
+        W yv = s_wv[r][ww];
+        combine<Op>(cf, cv, yf, yv);
+        cf = yf;
+        cv = yv;
+      }
+    }
+    const uint32_t agg_f = cf;
+    const W agg_v = cv;
+
+    // Tile prefix: ordered block-wide look-back.
+    uint32_t pf = 0;
+    W pv = id;
+    if (tile == 0) {
+      if (tid == 0 && p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, agg_f, agg_v);
+    } else {
+      // A tile that contains a segment start publishes its prefix right away:
+      // its aggregate is final for everything after it.
+      if (tid == 0) SIO::publish(p.status, tile, p.epoch, agg_f ? st_prefix : st_aggregate, agg_f, agg_v);
+      // Only elements before the tile's first segment start need the
+      // prefix; a tile that begins with a segment start needs none.
+      if (!head_at(0)) {
+        // thread t polls tile (base_t - (B - 1 - t)); thread B-1 = nearest.
+        int64_t wbase = static_cast<int64_t>(tile) - 1;
+        uint32_t ef = 0;  // accumulated prefix of tiles nearer than the window
+        W ev = id;
+        uint32_t polls = 0;
+        for (;;) {
+          const int64_t j = wbase - (B - 1 - tid);
+          uint32_t lf = 0;
+          W lv = id;
+          int q;
+          for (;;) {
+            uint32_t st = st_prefix;
+            lf = 0;
+            lv = id;
+            if (j >= 0) st = SIO::read(p.status, static_cast<uint32_t>(j), p.epoch, lf, lv);
+            if (++polls > lookback_poll_limit) {
+              if (tid == 0 && polls == lookback_poll_limit + 1)
+                __hip_atomic_fetch_add(&p.sync->watchdog_trips, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+              st = st_prefix;
+            }
+            const bool term = (st == st_prefix) || (st == st_aggregate && lf);
+            const uint64_t tm = wave::ballot(term);
+            const uint64_t im = wave::ballot(st == st_invalid);
+            if (lane == 0) {
+              s_q[w] = tm ? (w * 64 + 63 - __builtin_clzll(tm)) : -1;
+              s_invalid[w] = im;
+            }
+            __syncthreads();
+            q = -1;
+            for (int ww = nw - 1; ww >= 0; --ww)
+              if (s_q[ww] >= 0) {
+                q = s_q[ww];
+                break;
+              }
+            bool bad = false;
+            for (int ww = 0; ww < nw; ++ww) {
+              const int lo = ww * 64;
+              uint64_t need;
+              if (q <= lo) need = ~0ull;
+              else if (q > lo + 63) need = 0;
+              else need = ~0ull << (q - lo);
+              bad |= (s_invalid[ww] & need) != 0;
+            }
+            __syncthreads();
+            if (!bad) break;
+            __builtin_amdgcn_s_sleep(1);
+          }
+          // Fold window tiles [q .. B-1] in tile order.
+          uint32_t xf2 = (tid >= q) ? lf : 0;
+          W xv2 = (tid >= q) ? lv : id;
+          wave_pair_scan<Op>(xf2, xv2);
+          if (lane == 63) {
+            s_lbf[w] = xf2;
+            s_lbv[w] = xv2;
+          }
+          __syncthreads();
+          uint32_t wf = 0;
+          W wv = id;
+          for (int ww = 0; ww < nw; ++ww) {
+            uint32_t yf = s_lbf[ww];
+            W yv = s_lbv[ww];
+            combine<Op>(wf, wv, yf, yv);
+            wf = yf;
+            wv = yv;
+          }
+          __syncthreads();
+          // window (earlier) . accumulated (later)
+          combine<Op>(wf, wv, ef, ev);
+          if (q >= 0) break;
+          wbase -= B;
+        }
+        pf = ef;
+        pv = ev;
+        if (tid == 0 && !agg_f && tile + 1 < p.num_tiles) {
+          uint32_t of = agg_f;
+          W ov = agg_v;
+          combine<Op>(pf, pv, of, ov);
+          SIO::publish(p.status, tile, p.epoch, st_prefix, of, ov);
+        }
+      }
+    }
+
+    // Results.
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // P = tile prefix . before . wave-exclusive
+      uint32_t Pf = bf[r];
+      W Pv = bv[r];
+      {
+        uint32_t tf = xf[r];
+        W tv = xv[r];
+        combine<Op>(Pf, Pv, tf, tv);
+        Pf = tf;
+        Pv = tv;
+      }
+      {
+        uint32_t tf = Pf;
+        W tv = Pv;
+        combine<Op>(pf, pv, tf, tv);
+        Pf = tf;
+        Pv = tv;
+      }
+      W o[VEC];
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        // a segment start among elements 0..k of the vector cuts the prefix off
+        const bool cut = (fm[r] & ((2u << k) - 1u)) != 0u;
+        o[k] = cut ? v[r][k] : Op::apply(Pv, v[r][k]);
+      }
+      (void)Pf;
+      const uint64_t e0 = base + static_cast<uint64_t>(r * B + tid) * VEC;
+      if (!p.reduce) {
+        if (full) {
+          T ot[VEC];
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) ot[k] = static_cast<T>(o[k]);
+          u32x4 qv;
+          __builtin_memcpy(&qv, ot, 16);
+          reinterpret_cast<u32x4 *>(p.out + base)[r * B + tid] = qv;
+        } else {
+#pragma unroll
+          for (int k = 0; k < VEC; ++k)
+            if (e0 + k < p.total) p.out[e0 + k] = static_cast<T>(o[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+          const uint64_t e = e0 + k;
+          if (e >= p.total) continue;
+          const bool last = (e + 1 == p.total) || head_at(static_cast<uint64_t>(r * B + tid) * VEC + k + 1);
+          if (last) {
+            const uint64_t s = find_segment(p.offsets, s_lo, s_hi, e, p.total);
+            p.out[s] = static_cast<T>(Op::apply(p.init, o[k]));
+          }
+        }
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Host side
+// --------------------------------------------------------------------------
+// Four 16-byte rows per work-item keep every variant below 128 VGPRs; eight
+// rows spill and are only used when a caller asks for them.
+template <class T>
+int default_rows() {
+  return 4;
+}
+
+template <class T, class Op>
+struct launch_plan {
+  params<T> p;
+  uint32_t wg = 256;
+  int rows = 8;
+  uint32_t grid = 1;
+  size_t lds = 0;
+  uint64_t tile_elems = 0;
+  uint32_t *tile_seg = nullptr;
+  void *region = nullptr;
+  region_state *rs = nullptr;
+  size_t region_used = 0;
+  size_t region_cap = 0;
+};
+
+template <class T, class Op, int R>
+acpp_status launch_rows(launch_plan<T, Op> &lp, hipStream_t stream, const char *name) {
+  hipLaunchKernelGGL(setup_kernel<T>, dim3(static_cast<uint32_t>(std::min<uint64_t>(ceil_div(lp.p.num_segments + 1, 256), 4096))),
+                     dim3(256), 0, stream, lp.p, lp.tile_seg, lp.tile_elems);
+  acpp_status st = check_launch(name);
+  if (st != ACPP_SUCCESS) return st;
+  hipLaunchKernelGGL((segmented_kernel<T, Op, R>), dim3(lp.grid), dim3(lp.wg), lp.lds, stream, lp.p);
+  return check_launch(name);
+}
+
+template <class T, class Op>
+acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T *out,
+                    const uint64_t *offsets, uint64_t num_segments, uint64_t total,
+                    typename elem_traits<T>::work init, bool reduce,
+                    const acpp_launch_config *cfg, const char *name) {
+  constexpr int VEC = 16 / sizeof(T);
+  int cus = 0;
+  acpp_status st = enter_device(g, &cus);
+  if (st != ACPP_SUCCESS) return st;
+  launch_plan<T, Op> lp;
+  lp.wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 256;
+  if (!valid_work_group_size(lp.wg))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad work_group_size");
+  lp.rows = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows<T>();
+  if (lp.rows != 2 && lp.rows != 4 && lp.rows != 8)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": items_per_thread must be 2, 4 or 8");
+  lp.tile_elems = static_cast<uint64_t>(lp.wg) * lp.rows * VEC;
+  const uint64_t num_tiles = std::max<uint64_t>(1, ceil_div(total, lp.tile_elems));
+  if (num_tiles >= (1ull << 31) || num_segments >= (1ull << 32))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": problem too large");
+  lp.lds = static_cast<size_t>((lp.tile_elems + 1 + 31) / 32) * 4;
+
+  void *ts = nullptr;
+  st = group_obtain(g, (num_tiles + 1) * sizeof(uint32_t), &ts);
+  if (st != ACPP_SUCCESS) return st;
+  lp.tile_seg = static_cast<uint32_t *>(ts);
+  using SIO = seg_status<typename elem_traits<T>::work>;
+  lp.region_used = sizeof(sync_header) + num_tiles * SIO::words * sizeof(uint64_t);
+  st = group_obtain_sync(g, lp.region_used, stream, &lp.region, &lp.region_cap, &lp.rs);
+  if (st != ACPP_SUCCESS) return st;
+
+  const bool capturing = stream_is_capturing(stream);
+  uint32_t epoch = 1, ticket_base = 0;
+  if (capturing) {
+    st = fill_zero(lp.region, lp.region_used, stream);
+    if (st != ACPP_SUCCESS) return st;
+    lp.rs->mode = region_state::graph;
+  } else {
+    if (lp.rs->mode == region_state::graph || lp.rs->epoch >= seg_max_epoch) {
+      st = fill_zero(lp.region, lp.region_cap, stream);
+      if (st != ACPP_SUCCESS) return st;
+      lp.rs->epoch = 0;
+      lp.rs->ticket_base = 0;
+    }
+    lp.rs->mode = region_state::host_managed;
+    epoch = lp.rs->epoch + 1;
+    ticket_base = lp.rs->ticket_base;
+  }
+
+  params<T> &p = lp.p;
+  p.in = in;
+  p.out = out;
+  p.offsets = offsets;
+  p.tile_seg = lp.tile_seg;
+  p.num_segments = num_segments;
+  p.total = total;
+  p.init = init;
+  p.reduce = reduce ? 1 : 0;
+  p.vec_io = ((reinterpret_cast<uintptr_t>(in) | (reduce ? 0 : reinterpret_cast<uintptr_t>(out))) & 15) == 0;
+  p.num_tiles = static_cast<uint32_t>(num_tiles);
+  p.epoch = epoch;
+  p.ticket_base = ticket_base;
+  p.sync = static_cast<sync_header *>(lp.region);
+  p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(lp.region) + sizeof(sync_header));
+
+  // Persistent grid: one wave of resident work-groups (each draws tiles
+  // until the queue is empty).  Extra tickets drawn at the end are counted
+  // in the ticket base.
+  auto pick = [&](auto kernel) -> acpp_status {
+    int resident = 1;
+    ACPP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, static_cast<int>(lp.wg), lp.lds));
+    if (resident < 1) resident = 1;
+    uint64_t cap = static_cast<uint64_t>(resident) * cus;
+    if (cfg && cfg->groups_per_cu) cap = std::min<uint64_t>(cap, static_cast<uint64_t>(cfg->groups_per_cu) * cus);
+    lp.grid = static_cast<uint32_t>(std::min<uint64_t>(cap, num_tiles));
+    return ACPP_SUCCESS;
+  };
+  acpp_status s;
+  if (lp.rows == 2) {
+    if ((s = pick(segmented_kernel<T, Op, 2>)) != ACPP_SUCCESS) return s;
+    s = launch_rows<T, Op, 2>(lp, stream, name);
+  } else if (lp.rows == 4) {
+    if ((s = pick(segmented_kernel<T, Op, 4>)) != ACPP_SUCCESS) return s;
+    s = launch_rows<T, Op, 4>(lp, stream, name);
+  } else {
+    if ((s = pick(segmented_kernel<T, Op, 8>)) != ACPP_SUCCESS) return s;
+    s = launch_rows<T, Op, 8>(lp, stream, name);
+  }
+  if (s == ACPP_SUCCESS && !capturing) {
+    lp.rs->epoch = epoch;
+    // every work-group draws one ticket past the last tile before it exits
+    lp.rs->ticket_base = ticket_base + static_cast<uint32_t>(num_tiles) + lp.grid;
+  } else if (s != ACPP_SUCCESS) {
+    lp.rs->mode = region_state::graph;
+  }
+  return s;
+}
+
+template <class T>
+acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, T *out,
+                  const uint64_t *offsets, uint64_t num_segments, uint64_t total, T init, int op,
+                  bool reduce, const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (num_segments > 0 && !offsets)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": offsets is NULL");
+  if ((total > 0 && !in) || ((total > 0 || (reduce && num_segments > 0)) && !out))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": in or out is NULL");
+  if (op < ops::op_plus || op > ops::op_max)
+    return set_error(ACPP_ERROR_UNSUPPORTED, std::string(name) + ": segmented forms support plus, multiply, min and max");
+  if (num_segments == 0) return clear_error();
+  return with_op<W, T>(op, [&](auto opv) -> acpp_status {
+    using Op = decltype(opv);
+    if constexpr (Op::id > ops::op_max) {
+      return set_error(ACPP_ERROR_UNSUPPORTED, "unsupported operator");
+    } else {
+      acpp_status s = enqueue<T, Op>(g, stream, in, out, offsets, num_segments, total,
+                                     elem_traits<T>::to_work(init), reduce, cfg, name);
+      return s == ACPP_SUCCESS ? clear_error() : s;
+    }
+  });
+}
+
+}  // namespace seg_detail
+}  // namespace acpp_impl
+
+using namespace acpp_impl;
+using namespace acpp_impl::seg_detail;
+
+// --------------------------------------------------------------------------
+// Graph plans
+// --------------------------------------------------------------------------
+struct acpp_segmented_plan {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+};
+
+namespace {
+template <class F>
+acpp_status capture_plan(acpp_allocation_group *g, F &&enqueue_fn, acpp_segmented_plan **out) {
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "plan output pointer is NULL");
+  int cus = 0;
+  acpp_status st = enter_device(g, &cus);
+  if (st != ACPP_SUCCESS) return st;
+  hipStream_t cs = nullptr;
+  ACPP_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  std::unique_ptr<acpp_segmented_plan> plan(new acpp_segmented_plan);
+  plan->device = group_device(g);
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return hip_error(e, "hipStreamBeginCapture");
+  }
+  acpp_status s = enqueue_fn(cs);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(cs, &graph);
+  (void)hipStreamDestroy(cs);
+  if (s != ACPP_SUCCESS) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return s;
+  }
+  if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+  plan->graph = graph;
+  e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_error(e, "hipGraphInstantiate");
+  }
+  *out = plan.release();
+  return clear_error();
+}
+}  // namespace
+
+extern "C" {
+
+#define ACPP_DEFINE_SEGMENTED(tag, T)                                                                 \
+  acpp_status acpp_segmented_inclusive_scan_##tag(acpp_allocation_group *scratch, acpp_stream_t stream, \
+                                                   const T *first, uint64_t n, T *d_first,             \
+                                                   const uint64_t *offsets, uint64_t num_segments,     \
+                                                   acpp_algorithm_op op,                               \
+                                                   const acpp_launch_config *config) {                 \
+    return entry<T>(scratch, static_cast<hipStream_t>(stream), first, d_first, offsets, num_segments,  \
+                    n, T{}, op, false, config, "acpp_segmented_inclusive_scan_" #tag);                \
+  }                                                                                                   \
+  acpp_status acpp_segmented_reduce_##tag(acpp_allocation_group *scratch, acpp_stream_t stream,       \
+                                          const T *first, uint64_t n, const uint64_t *offsets,        \
+                                          uint64_t num_segments, T *out, T init,                      \
+                                          acpp_algorithm_op op, const acpp_launch_config *config) {   \
+    return entry<T>(scratch, static_cast<hipStream_t>(stream), first, out, offsets, num_segments, n,   \
+                    init, op, true, config, "acpp_segmented_reduce_" #tag);                           \
+  }                                                                                                   \
+  acpp_status acpp_segmented_plan_inclusive_scan_##tag(                                                \
+      acpp_allocation_group *scratch, const T *first, uint64_t n, T *d_first,                         \
+      const uint64_t *offsets, uint64_t num_segments, acpp_algorithm_op op,                           \
+      const acpp_launch_config *config, acpp_segmented_plan **plan) {                                 \
+    if (!scratch) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "scratch group is NULL");             \
+    return capture_plan(                                                                              \
+        scratch,                                                                                      \
+        [&](hipStream_t cs) {                                                                         \
+          return entry<T>(scratch, cs, first, d_first, offsets, num_segments, n, T{}, op, false,      \
+                          config, "acpp_segmented_plan_inclusive_scan_" #tag);                       \
+        },                                                                                            \
+        plan);                                                                                        \
+  }                                                                                                   \
+  acpp_status acpp_segmented_plan_reduce_##tag(                                                        \
+      acpp_allocation_group *scratch, const T *first, uint64_t n, const uint64_t *offsets,            \
+      uint64_t num_segments, T *out, T init, acpp_algorithm_op op, const acpp_launch_config *config, \
+      acpp_segmented_plan **plan) {                                                                   \
+    if (!scratch) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "scratch group is NULL");             \
+    return capture_plan(                                                                              \
+        scratch,                                                                                      \
+        [&](hipStream_t cs) {                                                                         \
+          return entry<T>(scratch, cs, first, out, offsets, num_segments, n, init, op, true, config,   \
+                          "acpp_segmented_plan_reduce_" #tag);                                        \
+        },                                                                                            \
+        plan);                                                                                        \
+  }
+
+ACPP_DEFINE_SEGMENTED(i32, int32_t)
+ACPP_DEFINE_SEGMENTED(i64, int64_t)
+ACPP_DEFINE_SEGMENTED(f32, float)
+ACPP_DEFINE_SEGMENTED(f64, double)
+
+acpp_status acpp_segmented_plan_launch(acpp_segmented_plan *plan, acpp_stream_t stream) {
+  if (!plan || !plan->exec) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "plan is NULL");
+  int cur = 0;
+  ACPP_CHECK_HIP(hipGetDevice(&cur));
+  if (cur != plan->device) ACPP_CHECK_HIP(hipSetDevice(plan->device));
+  ACPP_CHECK_HIP(hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream)));
+  return clear_error();
+}
+
+acpp_status acpp_segmented_plan_destroy(acpp_segmented_plan *plan) {
+  if (!plan) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "plan is NULL");
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  delete plan;
+  return clear_error();
+}
+
+}  // extern "C"

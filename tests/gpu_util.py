@@ -53,3 +53,11 @@ def zipf_keys(n, num_bins, s=1.1, seed=0):
     cdf /= cdf[-1]
     u = rng.random(n)
     return np.minimum(np.searchsorted(cdf, u, side="right"), num_bins - 1).astype(np.uint32)
+
+
+def lognormal_offsets(num_segments, median=512, sigma=2.0, max_len=1 << 20, seed=0, min_len=1):
+    """Segment offsets with log-normal lengths clip(round(exp(N(ln median, sigma))))."""
+    rng = np.random.default_rng(seed)
+    lens = np.clip(np.round(np.exp(rng.normal(np.log(median), sigma, num_segments))),
+                   min_len, max_len).astype(np.uint64)
+    return np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)

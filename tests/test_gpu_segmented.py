@@ -1,0 +1,154 @@
+"""GPU parity: segmented inclusive scans and reductions, direct and through
+hipGraph plans, against the oracle's per-segment serial scan / fold."""
+import math
+
+import numpy as np
+import pytest
+
+import opensycl_amd as acpp
+import oracle as O
+from gpu_util import dev, host, lognormal_offsets, rand
+
+pytestmark = pytest.mark.gpu
+
+
+def layouts():
+    rng = np.random.default_rng(5)
+    yield "lognormal", lognormal_offsets(3000, seed=1, max_len=1 << 18)
+    yield "ones", np.arange(0, 20001, dtype=np.uint64)
+    lens = rng.integers(0, 5, 5000).astype(np.uint64)  # many empty segments
+    yield "with_empty", np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    yield "one_huge", np.array([0, (1 << 22) + 77], dtype=np.uint64)
+    lens = np.array([0, 0, 3, 0, 70000, 0, 1, 123456, 0], dtype=np.uint64)
+    yield "mixed_edges", np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+
+
+def _values(dtype, n, seed):
+    if np.issubdtype(dtype, np.floating):
+        return rand(dtype, n, seed)
+    return rand(dtype, n, seed, lo=-(1 << 20), hi=1 << 20)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.float32, np.float64])
+def test_segmented_scan(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    for name, off in layouts():
+        n = int(off[-1])
+        x = _values(dtype, n, seed=n)
+        xd, od = dev(torch, x), dev(torch, off)
+        for op in (acpp.PLUS, acpp.MAX):
+            yd = torch.zeros_like(xd)
+            acpp.segmented_inclusive_scan(q, scratch, xd, yd, od, op)
+            q.wait()
+            got = host(yd)
+            if np.issubdtype(dtype, np.floating) and op == acpp.PLUS:
+                # FP64 per-segment prefix sums; fold depth bound per element
+                eps = np.finfo(dtype).eps
+                ok = True
+                x64 = x.astype(np.float64)
+                for s in range(len(off) - 1):
+                    b, e = int(off[s]), int(off[s + 1])
+                    if e > b:
+                        ex = np.cumsum(x64[b:e])
+                        bound = (64 + 2 * math.log2(e - b + 2)) * eps * np.cumsum(np.abs(x64[b:e]))
+                        ok &= bool(np.all(np.abs(got[b:e].astype(np.float64) - ex) <= bound + 1e-300))
+                assert ok, (name, op)
+            else:
+                assert np.array_equal(got, O.segmented_inclusive_scan(x, off, op)), (name, op)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.float32, np.float64])
+def test_segmented_reduce(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    for name, off in layouts():
+        n = int(off[-1])
+        S = len(off) - 1
+        x = _values(dtype, n, seed=n + 1)
+        xd, od = dev(torch, x), dev(torch, off)
+        for op, init in ((acpp.PLUS, dtype(7)), (acpp.MIN, dtype(1 << 20)), (acpp.MAX, dtype(-5))):
+            out = torch.full((S,), -99, dtype=xd.dtype, device="cuda")
+            acpp.segmented_reduce(q, scratch, xd, od, out, init, op)
+            q.wait()
+            got = host(out)
+            if np.issubdtype(dtype, np.floating) and op == acpp.PLUS:
+                exact = O.segmented_reduce_f32_fp64(x.astype(np.float32), off) if dtype == np.float32 \
+                    else np.array([math.fsum(x[int(off[s]):int(off[s + 1])]) for s in range(S)])
+                lens = np.diff(off.astype(np.int64))
+                absx = np.abs(x.astype(np.float64))
+                abs_sums = np.array([absx[int(off[s]):int(off[s + 1])].sum() for s in range(S)])
+                eps = np.finfo(dtype).eps
+                bound = (64 + 2 * np.log2(lens + 2)) * eps * (abs_sums + 7) + 1e-300
+                assert np.all(np.abs(got.astype(np.float64) - (exact + 7)) <= bound), name
+            else:
+                assert np.array_equal(got, O.segmented_reduce(x, off, init, op)), (name, op)
+
+
+@pytest.mark.parametrize("cfg", [(64, 2), (128, 4), (256, 2), (512, 4), (1024, 2), (1024, 4)])
+def test_segmented_configs(gpu, q, scratch, cfg):
+    torch, _ = gpu
+    off = lognormal_offsets(2000, seed=3, max_len=1 << 17)
+    n = int(off[-1])
+    x = rand(np.int64, n, seed=9, lo=-1000, hi=1000)
+    xd, od = dev(torch, x), dev(torch, off)
+    yd = torch.zeros_like(xd)
+    acpp.segmented_inclusive_scan(q, scratch, xd, yd, od, acpp.PLUS, config=cfg)
+    out = torch.zeros(len(off) - 1, dtype=torch.int64, device="cuda")
+    acpp.segmented_reduce(q, scratch, xd, od, out, np.int64(0), acpp.PLUS, config=cfg)
+    q.wait()
+    assert np.array_equal(host(yd), O.segmented_inclusive_scan(x, off))
+    assert np.array_equal(host(out), O.segmented_reduce(x, off, np.int64(0)))
+
+
+def test_segmented_plan_replays(gpu, q, scratch):
+    # hipGraph replays, each checked, with eager work between replays
+    torch, _ = gpu
+    off = lognormal_offsets(4000, seed=4, max_len=1 << 18)
+    n = int(off[-1])
+    x = rand(np.int64, n, seed=2, lo=-(1 << 20), hi=1 << 20)
+    xd, od = dev(torch, x), dev(torch, off)
+    yd = torch.zeros_like(xd)
+    f = rand(np.float32, n, seed=3)
+    fd = dev(torch, f)
+    red = torch.zeros(len(off) - 1, dtype=torch.float32, device="cuda")
+    scan_plan = acpp.SegmentedPlan.inclusive_scan(scratch, xd, yd, od)
+    red_plan = acpp.SegmentedPlan.reduce(scratch, fd, od, red, 0.0)
+    exp_scan = O.segmented_inclusive_scan(x, off)
+    exact = O.segmented_reduce_f32_fp64(f, off)
+    lens = np.diff(off.astype(np.int64))
+    abs_sums = O.segmented_reduce_f32_fp64(np.abs(f), off)
+    bound = (64 + 2 * np.log2(lens + 2)) * np.finfo(np.float32).eps * abs_sums + 1e-30
+    for rep in range(4):
+        yd.zero_()
+        red.zero_()
+        scan_plan.launch(q)
+        red_plan.launch(q)
+        q.wait()
+        assert np.array_equal(host(yd), exp_scan), rep
+        got = host(red).astype(np.float64)
+        assert np.all(np.abs(got - exact) <= bound), rep
+        # eager work on the same stream between replays
+        tmp = torch.zeros_like(xd)
+        acpp.inclusive_scan(q, scratch, xd, tmp)
+        q.wait()
+        # change the input: the next replay must see the new values
+        x = x + 1
+        xd.copy_(dev(torch, x))
+        exp_scan = O.segmented_inclusive_scan(x, off)
+    scan_plan.close()
+    red_plan.close()
+
+
+def test_segmented_lognormal_workload_shape(gpu, q, scratch):
+    # 65,536 segments, log-normal lengths (median 512, sigma 2, clipped to
+    # [1, 2^20]): int64 segmented inclusive scan, exact
+    torch, _ = gpu
+    off = lognormal_offsets(65536, seed=0)
+    n = int(off[-1])
+    x = rand(np.int64, n, seed=7, lo=-(1 << 20), hi=1 << 20)
+    xd, od = dev(torch, x), dev(torch, off)
+    yd = torch.empty_like(xd)
+    acpp.segmented_inclusive_scan(q, scratch, xd, yd, od)
+    q.wait()
+    y = host(yd)
+    del yd
+    assert np.array_equal(y, O.segmented_inclusive_scan(x, off))
