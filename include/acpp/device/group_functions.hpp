@@ -31,50 +31,12 @@
 #include <type_traits>
 
 #include "acpp/device/ops.hpp"
+#include "acpp/functional.hpp"
 #include "acpp/device/wave.hpp"
 
 namespace acpp {
 
-// ---------------------------------------------------------------------------
-// Operators (sycl::plus, multiplies, minimum, maximum, bit_*, logical_*;
-// include/hipSYCL/sycl/libkernel/functional.hpp).
-// ---------------------------------------------------------------------------
-template <class T = void>
-struct plus {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return a + b; }
-};
-template <class T = void>
-struct multiplies {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return a * b; }
-};
-template <class T = void>
-struct minimum {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return (a < b) ? a : b; }
-};
-template <class T = void>
-struct maximum {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return (a > b) ? a : b; }
-};
-template <class T = void>
-struct bit_and {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return a & b; }
-};
-template <class T = void>
-struct bit_or {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return a | b; }
-};
-template <class T = void>
-struct bit_xor {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return a ^ b; }
-};
-template <class T = void>
-struct logical_and {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return T(a && b); }
-};
-template <class T = void>
-struct logical_or {
-  __host__ __device__ T operator()(const T &a, const T &b) const { return T(a || b); }
-};
+// Operators: acpp::plus ... acpp::logical_or (acpp/functional.hpp).
 
 // Known identities (sycl::known_identity, functional.hpp:123-244).
 template <class Op, class T>

@@ -130,6 +130,25 @@ def build_bench_helpers() -> str:
     return out
 
 
+def build_cpp_tests() -> str:
+    """Build tests/lib/test_facade: the C++ facade test (host code only,
+    compiled with g++ against the library and the HIP runtime)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+    out_dir = os.path.join(ROOT, "tests", "lib")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "test_facade")
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", f"-I{rocm}/include",
+           "-D__HIP_PLATFORM_AMD__", src, "-o", out + ".tmp", f"-L{LIB_DIR}", "-lacpp_mi355x",
+           f"-L{rocm}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../opensycl_amd/lib",
+           f"-Wl,-rpath,{rocm}/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C++ facade test build failed:\n" + r.stderr)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle() -> str:
     """Build the CPU oracle (test infrastructure) with its Makefile."""
     d = os.path.join(ROOT, "oracle")

@@ -261,6 +261,9 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   }
   __syncthreads();
   const uint32_t tile = s_tile;
+  // Tickets beyond this launch's tiles only appear if the host-side ticket
+  // count went out of step with the counter; never touch memory for them.
+  if (tile >= p.num_tiles) return;
   const uint32_t epoch = p.epoch;
 
   const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;

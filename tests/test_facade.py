@@ -1,0 +1,73 @@
+"""C++ facade (include/acpp/algorithms/numeric.hpp): builds against the
+library, rejects what the library does not provide at compile time, and
+(on a GPU) runs tests/cpp/test_facade.cpp."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+BIN = os.path.join(ROOT, "tests", "lib", "test_facade")
+
+
+def _syntax(code, tmp_path):
+    src = tmp_path / "snippet.cpp"
+    src.write_text(code)
+    return subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{INCLUDE}", "-I/opt/rocm/include",
+                           "-D__HIP_PLATFORM_AMD__", str(src)], capture_output=True, text=True)
+
+
+PRELUDE = """
+#include "acpp/algorithms/numeric.hpp"
+namespace alg = acpp::algorithms;
+void f(acpp::queue& q, alg::util::allocation_group& g, int* x, int* y, float* fx, long* lx) {
+"""
+
+
+def test_facade_signatures_compile(tmp_path):
+    # The reference's call forms (numeric.hpp:170-297) must all compile.
+    r = _syntax(PRELUDE + """
+  std::vector<acpp::event> deps;
+  alg::reduce(q, g, x, x + 10, y);
+  alg::reduce(q, g, x, x + 10, y, 3);
+  alg::reduce(q, g, x, x + 10, y, 3, std::plus<int>{});
+  alg::reduce(q, g, x, x + 10, y, 3, acpp::maximum<int>{}, deps);
+  alg::reduce(q, g, fx, fx + 10, fx, 0.f, acpp::minimum<float>{});
+  alg::transform_reduce(q, g, fx, fx + 10, fx, fx, 0.f);
+  alg::transform_reduce(q, g, fx, fx + 10, fx, fx, 0.f, std::plus<float>{}, std::multiplies<float>{});
+  alg::inclusive_scan(q, g, x, x + 10, y);
+  alg::inclusive_scan(q, g, x, x + 10, y, std::bit_xor<int>{});
+  alg::inclusive_scan(q, g, x, x + 10, y, acpp::plus<int>{}, 7, deps);
+  alg::exclusive_scan(q, g, x, x + 10, y, 0);
+  alg::exclusive_scan(q, g, lx, lx + 10, lx, 0L, acpp::maximum<long>{});
+}
+""", tmp_path)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("call,message", [
+    ("alg::reduce(q, g, x, x + 10, y, 0, [](int a, int b) { return a + b; });", "operator must be"),
+    ("alg::reduce(q, g, fx, fx + 10, fx, 0.f, std::bit_and<float>{});", "floating-point types take"),
+    ("alg::reduce(q, g, fx, fx + 10, fx, 0.f, acpp::bit_or<float>{});", "floating-point types take"),
+    ("alg::transform_reduce(q, g, x, x + 10, x, y, 0, acpp::maximum<int>{}, std::multiplies<int>{});",
+     "reduce = plus with"),
+    ("alg::reduce(q, g, x, x + 10, lx, 0L);", "input value type must equal T"),
+])
+def test_facade_rejects_at_compile_time(tmp_path, call, message):
+    # No generic-functor fallback: what the library lacks fails to compile.
+    r = _syntax(PRELUDE + call + "\n}\n", tmp_path)
+    assert r.returncode != 0
+    assert message in r.stderr, r.stderr[-2000:]
+
+
+def test_facade_binary_built():
+    assert os.path.exists(BIN), "run __graft_entry__.build() first"
+
+
+@pytest.mark.gpu
+def test_facade_on_gpu():
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout
