@@ -307,6 +307,36 @@ acpp_status acpp_segmented_plan_reduce_f64(acpp_allocation_group *scratch, const
 acpp_status acpp_segmented_plan_launch(acpp_segmented_plan *plan, acpp_stream_t stream);
 acpp_status acpp_segmented_plan_destroy(acpp_segmented_plan *plan);
 
+/* ------------------------------------------------------------------------ */
+/* Learned cost model (no counterpart in the reference, whose launch        */
+/* geometry is fixed: numeric.hpp:144-156, decoupled_lookback_scan.hpp:     */
+/* 481-488).  A 16 -> 64 -> 64 -> 1 tanh network that predicts the          */
+/* standardised log run time of a launch configuration on a problem shape. */
+/* Parameters: one float array, layout W1[64][16], b1[64], W2[64][64],      */
+/* b2[64], w3[64], b3.  Inputs x: [batch][16] row-major.  All pointers are  */
+/* device pointers; batch in [1, 2^20].                                     */
+/* ------------------------------------------------------------------------ */
+#define ACPP_MLP_INPUTS 16
+#define ACPP_MLP_HIDDEN 64
+#define ACPP_MLP_PARAMS 5313
+
+/* y[b] = network(x[b]) */
+acpp_status acpp_mlp_forward(acpp_stream_t stream, const float *params, const float *x, uint32_t batch,
+                             float *y);
+/* grad = dL/dparams of L = mean_b (y[b] - target[b])^2; *loss = L if loss
+   is not NULL.  Deterministic: the same inputs give the same bits. */
+acpp_status acpp_mlp_gradient(acpp_allocation_group *scratch, acpp_stream_t stream, const float *params,
+                              const float *x, const float *target, uint32_t batch, float *grad,
+                              float *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, L2 weight decay added to
+   the gradient) on the full batch; `step` counts from 1 for bias
+   correction; adam_m / adam_v hold the moment estimates (zero before step
+   1).  *loss = L before the update if loss is not NULL. */
+acpp_status acpp_mlp_adam_step(acpp_allocation_group *scratch, acpp_stream_t stream, float *params,
+                               float *adam_m, float *adam_v, uint32_t step, const float *x,
+                               const float *target, uint32_t batch, float lr, float weight_decay,
+                               float *loss);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -11,5 +11,6 @@ from .algorithms import (ALLOCATION_DEVICE, ALLOCATION_HOST, ALLOCATION_SHARED, 
                          PLUS, AllocationCache, AllocationGroup, Queue, compute_units,
                          exclusive_scan, inclusive_scan, reduce, reduce_compensated, reduction,
                          transform_reduce)
+from .cost_model import CostModel  # noqa: F401
 
 __version__ = "0.1.0"

@@ -85,8 +85,16 @@ def _signatures() -> dict[str, tuple[list, object]]:
         for tag in ("i32", "u32", "i64", "u64", "f32", "f64"):
             S[f"acpp_atomic_fetch_{op}_strided_{tag}"] = (
                 [vp, vp, vp, c_uint64, vp, c_uint32, c_int, cfg], c_int)
+    S["acpp_mlp_forward"] = ([vp, vp, vp, c_uint32, vp], c_int)
+    S["acpp_mlp_gradient"] = ([vp, vp, vp, vp, vp, c_uint32, vp, vp], c_int)
+    S["acpp_mlp_adam_step"] = ([vp, vp, vp, vp, vp, c_uint32, vp, vp, c_uint32, c_float, c_float, vp],
+                               c_int)
     return S
 
+
+MLP_INPUTS = 16
+MLP_HIDDEN = 64
+MLP_PARAMS = 5313
 
 SIGNATURES = _signatures()
 
