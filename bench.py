@@ -1,23 +1,40 @@
 #!/usr/bin/env python3
 """Benchmark of the device-wide primitives on one MI355X.
 
-Workload (the bandwidth-scale configuration):
+Headline workload (one step):
   exclusive_scan(plus, init 0) and reduce(plus, init 0) over n = 2^28 int32
   drawn uniform in [-2^15, 2^15], plus transform_reduce (dot product) of two
-  2^28 fp32 vectors drawn N(0,1).  One step = one pass of the three calls.
+  2^28 fp32 vectors drawn N(0,1).
 
-value = algorithmic bytes of one step x steps / wall time of the timed steps
-        (inputs resident in HBM), in GB/s.
-Algorithmic bytes per call: exclusive_scan 8n, reduce 4n, dot 8n.
+value = algorithmic bytes of one step x steps / time of the timed steps
+        (inputs resident in HBM), in GB/s.  Algorithmic bytes per call:
+        exclusive_scan 8n, reduce 4n, dot 8n.
 
-The roofline object describes the dominant kernel (the scan): achieved GB/s
-from HIP events recorded around every scan launch inside the timed region.
-The cpu_baseline object times the oracle's restatement of the reference's
-OpenMP-backend algorithms on this host's cores for the same workload and
-checks the GPU results against it.
+Other fields:
+  roofline      -- the dominant kernel (the scan): algorithmic bytes per
+                   launch / average launch time (HIP events on the stream the
+                   kernel runs on, recorded around every launch in the timed
+                   region), against the 8 TB/s HBM peak; traffic = HBM bytes
+                   per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+                   (profiles/traffic.json) when present;
+  segmented     -- 65,536 segments with log-normal lengths (median 512,
+                   sigma 2, clipped to [1, 2^20]): int64 segmented inclusive
+                   scan and fp32 segmented reduce, one hipGraph plan launch
+                   (and one direct launch) against one plain call per segment
+                   issued from native code;
+  other_configs -- compensated / plain fp32 and bf16 sums and dot products
+                   with 1% of entries scaled by 1e4 (relative error against
+                   a float64 torch sum), histograms of Zipf(1.1) keys into
+                   4096 and 65536 bins and fetch_max into 1024 slots, with
+                   LDS-privatised and direct global atomics;
+  cpu_baseline  -- the oracle's OpenMP restatement of the reference's
+                   OpenMP-backend algorithms on this host's cores, on the
+                   headline workload, with the GPU results checked against it.
 """
 import argparse
+import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -33,6 +50,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def timed(q, fn, reps, release=None):
+    """Average and minimum time (us) of `reps` calls of fn, HIP events on
+    the queue's stream around each call."""
+    import torch
+    out = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(q.stream)
+        fn()
+        e1.record(q.stream)
+        if release:
+            release()
+        out.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b in out]
+    return sum(ms) / len(ms) * 1e3, min(ms) * 1e3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,8 +76,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-segmented", action="store_true", help="skip the segmented leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the precision / atomics configs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain device copy of the int32 input (profiling calibration)")
     ap.add_argument("--ops", default="scan,reduce,dot",
-                    help="comma-separated subset of scan,reduce,dot (profiling runs)")
+                    help="comma-separated subset of scan,reduce,dot")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -62,8 +102,7 @@ def main():
 
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0)
-    x = torch.randint(-(1 << 15), (1 << 15) + 1, (n,), dtype=torch.int32, device="cuda",
-                      generator=gen)
+    x = torch.randint(-(1 << 15), (1 << 15) + 1, (n,), dtype=torch.int32, device="cuda", generator=gen)
     y = torch.empty_like(x)
     gen.manual_seed(1)
     a = torch.randn(n, dtype=torch.float32, device="cuda", generator=gen)
@@ -85,14 +124,20 @@ def main():
             acpp.reduce(q, scratch, x, red_out, np.int32(0), acpp.PLUS)
         else:
             acpp.transform_reduce(q, scratch, a, b, dot_out, 0.0)
+        # every call is ordered on one stream: its scratch can go back to
+        # the cache at once (allocation_cache.hpp:129-141)
+        scratch.release()
 
     for _ in range(args.warmup):
         for op in ops:
             run_op(op)
+    if args.calibrate_copy:
+        for _ in range(3):
+            y.copy_(x)
     torch.cuda.synchronize()
     log("warmup done")
 
-    stream = torch.cuda.current_stream()
+    stream = q.stream
     ev = {op: [] for op in ops}
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
@@ -121,6 +166,7 @@ def main():
             "algorithmic_bytes": bytes_per[op],
             "gbs": round(bytes_per[op] / (avg * 1e-3) / 1e9, 1),
             "frac_of_peak": round(bytes_per[op] / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "elements_per_s": round(n / (avg * 1e-3), 1),
         }
 
     step_bytes = sum(bytes_per[op] for op in ops)
@@ -132,12 +178,13 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get(f"{dom}_2^{args.log2n}")
-        except Exception:
+            with open(tfile) as f:
+                traffic = json.load(f).get(f"{dom}_2^{args.log2n}", {}).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
             traffic = None
     roofline = {
-        "kernel": {"scan": "scan_kernel<int32, plus, 8 rows>",
-                   "reduce": "reduce_kernel<int32, plus>",
+        "kernel": {"scan": "scan_kernel<int, plus, 8 rows> (1024 work-items)",
+                   "reduce": "reduce_kernel<int, plus>",
                    "dot": "reduce_kernel<float, dot>"}[dom],
         "bound": "hbm",
         "achieved": per_op[dom]["gbs"],
@@ -145,6 +192,8 @@ def main():
         "unit": "GB/s",
         "frac": round(per_op[dom]["gbs"] / PEAK_HBM_GBS, 4),
         "traffic": traffic,
+        "algorithmic_bytes_per_launch": bytes_per[dom],
+        "avg_launch_us": per_op[dom]["avg_us"],
     }
 
     result = {
@@ -169,13 +218,32 @@ def main():
         },
         "ops": per_op,
         "roofline": roofline,
-        "segmented_speedup": None,
     }
 
-    # GPU results of the last step, checked below against the CPU leg.
     torch.cuda.synchronize()
+    gpu_results = {"y": y, "red": red_out, "dot": dot_out}
+
+    if not args.no_segmented:
+        try:
+            result["segmented"] = segmented_leg(acpp, q, cache)
+            result["segmented_speedup"] = {k: v["speedup_graph_vs_per_segment"]
+                                           for k, v in result["segmented"].items() if isinstance(v, dict)}
+        except Exception as e:  # report, do not hide the headline numbers
+            log(f"segmented leg failed: {e!r}")
+            result["segmented"] = {"error": repr(e)}
+            result["segmented_speedup"] = None
+    else:
+        result["segmented_speedup"] = None
+
+    if not args.no_extras:
+        try:
+            result["other_configs"] = extras_leg(acpp, q, cache, args.log2n)
+        except Exception as e:
+            log(f"extras leg failed: {e!r}")
+            result["other_configs"] = {"error": repr(e)}
+
     if not args.skip_cpu:
-        result["cpu_baseline"] = cpu_baseline(x, y, a, b, red_out, dot_out, ops, bytes_per)
+        result["cpu_baseline"] = cpu_baseline(x, gpu_results, a, b, ops, bytes_per)
     else:
         result["cpu_baseline"] = None
     scratch.close()
@@ -183,7 +251,204 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(x, y, a, b, red_out, dot_out, ops, bytes_per):
+def segmented_leg(acpp, q, cache):
+    """65,536 log-normal segments: batched (graph plan / direct) against one
+    plain call per segment from native code (opensycl_amd/lib/libacpp_bench.so)."""
+    import numpy as np
+    import torch
+
+    from opensycl_amd import tuner
+
+    shape = tuner.lognormal_workload("scan", "i64", S=65536, seed=0)
+    off_h = shape.offsets()
+    T = int(off_h[-1])
+    S = len(off_h) - 1
+    log(f"segmented: {S} segments, {T} elements")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    xs = torch.randint(-(1 << 20), (1 << 20) + 1, (T,), dtype=torch.int64, device="cuda", generator=gen)
+    ys = torch.empty_like(xs)
+    yref = torch.empty_like(xs)
+    gen.manual_seed(6)
+    xf = torch.randn(T, dtype=torch.float32, device="cuda", generator=gen)
+    rs = torch.empty(S, dtype=torch.float32, device="cuda")
+    rref = torch.empty(S, dtype=torch.float32, device="cuda")
+    off = torch.from_numpy(off_h).cuda()
+    off_host = np.ascontiguousarray(off_h)
+    bench_lib = ctypes.CDLL(os.path.join(ROOT, "opensycl_amd", "lib", "libacpp_bench.so"))
+    bench_lib.acpp_bench_per_segment_inclusive_scan_i64.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_uint64]
+    bench_lib.acpp_bench_per_segment_reduce_f32.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+        ctypes.c_void_p, ctypes.c_float]
+    torch.cuda.synchronize()
+    out = {"workload": "65,536 segments, log-normal lengths (median 512, sigma 2, clipped to [1, 2^20]); "
+                       "int64 values U[-2^20, 2^20], fp32 values N(0,1)",
+           "segments": S, "elements": T}
+    with acpp.AllocationGroup(cache, 0) as g:
+        # --- int64 segmented inclusive scan
+        plan = acpp.SegmentedPlan.inclusive_scan(g, xs, ys, off)
+        plan.launch(q)
+        graph_avg, graph_min = timed(q, lambda: plan.launch(q), 10)
+
+        with acpp.AllocationGroup(cache, 0) as g2:
+            def direct2():
+                acpp.segmented_inclusive_scan(q, g2, xs, ys, off)
+            direct2()
+            g2.release()
+            direct_avg, _ = timed(q, direct2, 10, release=g2.release)
+
+        def per_segment():
+            st = bench_lib.acpp_bench_per_segment_inclusive_scan_i64(
+                cache.handle, 0, q.handle, xs.data_ptr(), yref.data_ptr(), off_host.ctypes.data, S)
+            if st != 0:
+                raise RuntimeError(f"per-segment scan failed: {st}")
+        per_segment()
+        per_avg, _ = timed(q, per_segment, 2)
+        plan.launch(q)
+        torch.cuda.synchronize()
+        exact = bool(torch.equal(ys, yref))
+        b_scan = 16 * T + 8 * (S + 1)
+        out["scan_i64"] = {
+            "batched_graph_us": round(graph_avg, 1), "batched_graph_min_us": round(graph_min, 1),
+            "batched_direct_us": round(direct_avg, 1), "per_segment_us": round(per_avg, 1),
+            "speedup_graph_vs_per_segment": round(per_avg / graph_avg, 1),
+            "algorithmic_bytes": b_scan, "gbs": round(b_scan / (graph_avg * 1e-6) / 1e9, 1),
+            "bit_exact_vs_per_segment": exact,
+        }
+        plan.close()
+        g.release()
+        # --- fp32 segmented reduce
+        plan = acpp.SegmentedPlan.reduce(g, xf, off, rs, 0.0)
+        plan.launch(q)
+        graph_avg, graph_min = timed(q, lambda: plan.launch(q), 10)
+
+        def per_segment_r():
+            st = bench_lib.acpp_bench_per_segment_reduce_f32(
+                cache.handle, 0, q.handle, xf.data_ptr(), off_host.ctypes.data, S, rref.data_ptr(),
+                ctypes.c_float(0.0))
+            if st != 0:
+                raise RuntimeError(f"per-segment reduce failed: {st}")
+        per_segment_r()
+        per_avg, _ = timed(q, per_segment_r, 2)
+        plan.launch(q)
+        torch.cuda.synchronize()
+        exact64 = torch.zeros(S, dtype=torch.float64, device="cuda")
+        seg_id = torch.repeat_interleave(torch.arange(S, device="cuda"),
+                                         torch.from_numpy(np.diff(off_h.astype(np.int64))).cuda())
+        exact64.index_add_(0, seg_id, xf.double())
+        abs64 = torch.zeros(S, dtype=torch.float64, device="cuda")
+        abs64.index_add_(0, seg_id, xf.double().abs())
+        err = ((rs.double() - exact64).abs() / abs64.clamp_min(1e-30)).max().item()
+        err_ref = ((rref.double() - exact64).abs() / abs64.clamp_min(1e-30)).max().item()
+        b_red = 4 * T + 8 * (S + 1) + 4 * S
+        out["reduce_f32"] = {
+            "batched_graph_us": round(graph_avg, 1), "batched_graph_min_us": round(graph_min, 1),
+            "per_segment_us": round(per_avg, 1),
+            "speedup_graph_vs_per_segment": round(per_avg / graph_avg, 1),
+            "algorithmic_bytes": b_red, "gbs": round(b_red / (graph_avg * 1e-6) / 1e9, 1),
+            "max_err_over_abs_sum_vs_fp64": err, "per_segment_max_err_over_abs_sum_vs_fp64": err_ref,
+        }
+        plan.close()
+    del xs, ys, yref, xf
+    torch.cuda.empty_cache()
+    log(f"segmented: {out}")
+    return out
+
+
+def extras_leg(acpp, q, cache, log2n):
+    """Precision form (compensated sums / dots) and atomics configurations."""
+    import numpy as np
+    import torch
+
+    n = 1 << log2n
+    res = {}
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(10)
+    f = torch.randn(n, dtype=torch.float32, device="cuda", generator=gen)
+    g2 = torch.randn(n, dtype=torch.float32, device="cuda", generator=gen)
+    idx = torch.randperm(n, device="cuda", generator=gen)[: n // 100]
+    f[idx] *= 1e4
+    fb = f.to(torch.bfloat16)
+    gb = g2.to(torch.bfloat16)
+    out = torch.zeros(1, dtype=torch.float32, device="cuda")
+    sum64 = f.double().sum().item()
+    abs64 = f.double().abs().sum().item()
+    sumb64 = fb.double().sum().item()
+    dot64 = (f.double() * g2.double()).sum().item()
+    dotb64 = (fb.double() * gb.double()).sum().item()
+    with acpp.AllocationGroup(cache, 0) as g:
+        cases = {
+            "sum_f32_plain": (lambda: acpp.reduce(q, g, f, out, 0.0), 4 * n, sum64),
+            "sum_f32_compensated": (lambda: acpp.reduce_compensated(q, g, f, out, 0.0), 4 * n, sum64),
+            "sum_bf16_plain": (lambda: acpp.reduce(q, g, fb, out, 0.0), 2 * n, sumb64),
+            "sum_bf16_compensated": (lambda: acpp.reduce_compensated(q, g, fb, out, 0.0), 2 * n, sumb64),
+            "dot_f32_plain": (lambda: acpp.transform_reduce(q, g, f, g2, out, 0.0), 8 * n, dot64),
+            "dot_f32_compensated": (lambda: acpp.transform_reduce(q, g, f, g2, out, 0.0, compensated=True),
+                                    8 * n, dot64),
+            "dot_bf16_plain": (lambda: acpp.transform_reduce(q, g, fb, gb, out, 0.0), 4 * n, dotb64),
+            "dot_bf16_compensated": (lambda: acpp.transform_reduce(q, g, fb, gb, out, 0.0, compensated=True),
+                                     4 * n, dotb64),
+        }
+        for name, (fn, nbytes, exact) in cases.items():
+            fn()
+            g.release()
+            avg, mn = timed(q, fn, 10, release=g.release)
+            val = out.item()
+            res[name] = {"avg_us": round(avg, 2), "gbs": round(nbytes / (avg * 1e-6) / 1e9, 1),
+                         "rel_err_vs_fp64": abs(val - exact) / max(abs(exact), 1e-300),
+                         "err_over_abs_sum": abs(val - exact) / abs64}
+        del fb, gb, g2
+        # Zipf(1.1) keys by inverse CDF on the device
+        for bins in (4096, 65536):
+            w = 1.0 / np.arange(1, bins + 1, dtype=np.float64) ** 1.1
+            cdf = torch.from_numpy(np.cumsum(w) / w.sum()).cuda()
+            u = torch.rand(n, dtype=torch.float64, device="cuda", generator=gen)
+            keys = torch.searchsorted(cdf, u).clamp_(max=bins - 1).to(torch.int32).view(torch.uint32)
+            del u
+            hist = {}
+            for strat_name, strat in (("privatized", acpp.ATOMICS_PRIVATIZED), ("global", acpp.ATOMICS_GLOBAL)):
+                h32 = torch.zeros(bins, dtype=torch.int32, device="cuda")
+                h = h32.view(torch.uint32)
+
+                def run():
+                    h32.zero_()
+                    acpp.histogram(q, g, keys, h, strategy=strat)
+                run()
+                g.release()
+                avg, _ = timed(q, run, 5, release=g.release)
+                hist[strat_name] = h32.clone()
+                res[f"histogram_{bins}_{strat_name}"] = {"avg_us": round(avg, 2),
+                                                         "gbs": round(4 * n / (avg * 1e-6) / 1e9, 1)}
+            same = bool(torch.equal(hist["privatized"], hist["global"]))
+            total = int(hist["privatized"].to(torch.int64).sum().item())
+            res[f"histogram_{bins}_check"] = {"strategies_bit_identical": same, "sum_equals_n": total == n}
+            del keys
+        vals = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32, device="cuda", generator=gen)
+        tmax = {}
+        for strat_name, strat in (("privatized", acpp.ATOMICS_PRIVATIZED), ("global", acpp.ATOMICS_GLOBAL)):
+            t = torch.full((1024,), -(1 << 31), dtype=torch.int32, device="cuda")
+
+            def runm():
+                t.fill_(-(1 << 31))
+                acpp.atomic_fetch_strided(q, g, "max", vals, t, strategy=strat)
+            runm()
+            g.release()
+            avg, _ = timed(q, runm, 5, release=g.release)
+            tmax[strat_name] = t.clone()
+            res[f"fetch_max_1024_{strat_name}"] = {"avg_us": round(avg, 2),
+                                                   "gbs": round(4 * n / (avg * 1e-6) / 1e9, 1)}
+        ref = vals.view(-1, 1024).max(0).values if n % 1024 == 0 else None
+        res["fetch_max_1024_check"] = {
+            "strategies_bit_identical": bool(torch.equal(tmax["privatized"], tmax["global"])),
+            "equals_torch_max": bool(ref is not None and torch.equal(tmax["privatized"], ref))}
+    torch.cuda.synchronize()
+    log(f"other configs: {res}")
+    return res
+
+
+def cpu_baseline(x, gpu, a, b, ops, bytes_per):
     """Time the oracle's OpenMP-order restatement of the reference's CPU path
     on the same data (the full 2^28 workload), and check the GPU results."""
     import numpy as np
@@ -201,19 +466,19 @@ def cpu_baseline(x, y, a, b, red_out, dot_out, ops, bytes_per):
         t0 = time.perf_counter()
         yc = O.scan_omp_order(xh, True, np.int32(0), O.PLUS, threads)
         times["scan"] = time.perf_counter() - t0
-        checks["scan_bit_exact"] = bool(np.array_equal(y.cpu().numpy(), yc))
+        checks["scan_bit_exact"] = bool(np.array_equal(gpu["y"].cpu().numpy(), yc))
         del yc
     if "reduce" in ops:
         t0 = time.perf_counter()
         rc = O.reduce_omp_order(xh, np.int32(0), O.PLUS, threads)
         times["reduce"] = time.perf_counter() - t0
-        checks["reduce_bit_exact"] = bool(int(red_out.item()) == int(rc))
+        checks["reduce_bit_exact"] = bool(int(gpu["red"].item()) == int(rc))
     if "dot" in ops:
         t0 = time.perf_counter()
         dc = O.dot_omp_order(ah, bh, np.float32(0), threads)
         times["dot"] = time.perf_counter() - t0
         exact = O.dot_fp64(ah, bh)
-        checks["dot_rel_err_vs_fp64_gpu"] = abs(dot_out.item() - exact) / max(abs(exact), 1e-30)
+        checks["dot_rel_err_vs_fp64_gpu"] = abs(gpu["dot"].item() - exact) / max(abs(exact), 1e-30)
         checks["dot_rel_err_vs_fp64_cpu"] = abs(float(dc) - exact) / max(abs(exact), 1e-30)
     total_bytes = sum(bytes_per[o] for o in times)
     total_t = sum(times.values())

@@ -24,10 +24,15 @@
 //     values, each carrying the tag), written and read with relaxed
 //     device-scope atomics.  Words from earlier launches carry an older
 //     epoch and read as "invalid", so nothing has to be cleared;
-//   - the look-back is done by the whole work-group: blockDim.x
-//     predecessors are polled at once, the nearest "prefix" among them is
-//     found with ballots, and the window is reduced with DPP and one LDS
-//     exchange (a one-wave, 64-tile window is kept as a variant);
+//   - each tile's status has a 128-byte line of its own.  Every tile in
+//     flight polls the statuses of its predecessors, so packed statuses
+//     put all those polls on a handful of lines; on MI355X spreading them
+//     out took the 2^28 int32 scan from ~490 us to ~410 us;
+//   - the look-back is done by one wave: 64 predecessors are polled at
+//     once, the nearest "prefix" among them is found with a ballot and the
+//     window is folded with DPP, with no work-group barrier per poll.
+//     A work-group-wide window (blockDim.x tiles per poll) is kept as a
+//     variant;
 //   - nothing is reset at the end of a launch: the host tracks, per scratch
 //     region, the epoch of the last launch and how many tile tickets have
 //     been drawn, and passes both as kernel arguments (a per-tile "done"
@@ -57,17 +62,21 @@ __device__ __forceinline__ void atomic_store_u64(uint64_t *p, uint64_t v) {
 template <class W, int Bytes = sizeof(W)>
 struct status_io;
 
+// 64-bit words per tile status slot: one 128-byte line.
+constexpr int status_stride_words = 16;
+
 template <class W>
 struct status_io<W, 4> {
   static constexpr int words = 1;
   __device__ __forceinline__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch,
                                                  uint32_t flag, W v) {
     const uint64_t tag = (static_cast<uint64_t>(epoch) << 2) | flag;
-    atomic_store_u64(st + tile, (tag << 32) | __builtin_bit_cast(uint32_t, v));
+    atomic_store_u64(st + static_cast<uint64_t>(tile) * status_stride_words,
+                     (tag << 32) | __builtin_bit_cast(uint32_t, v));
   }
   __device__ __forceinline__ static uint32_t read(const uint64_t *st, uint32_t tile,
                                                   uint32_t epoch, W &v) {
-    const uint64_t w = atomic_load_u64(st + tile);
+    const uint64_t w = atomic_load_u64(st + static_cast<uint64_t>(tile) * status_stride_words);
     const uint32_t tag = static_cast<uint32_t>(w >> 32);
     if ((tag >> 2) != epoch) return st_invalid;
     v = __builtin_bit_cast(W, static_cast<uint32_t>(w));
@@ -82,13 +91,15 @@ struct status_io<W, 8> {
                                                  uint32_t flag, W v) {
     const uint64_t tag = (static_cast<uint64_t>(epoch) << 2) | flag;
     const uint64_t bits = __builtin_bit_cast(uint64_t, v);
-    atomic_store_u64(st + 2 * static_cast<uint64_t>(tile), (tag << 32) | (bits & 0xffffffffu));
-    atomic_store_u64(st + 2 * static_cast<uint64_t>(tile) + 1, (tag << 32) | (bits >> 32));
+    uint64_t *slot = st + static_cast<uint64_t>(tile) * status_stride_words;
+    atomic_store_u64(slot, (tag << 32) | (bits & 0xffffffffu));
+    atomic_store_u64(slot + 1, (tag << 32) | (bits >> 32));
   }
   __device__ __forceinline__ static uint32_t read(const uint64_t *st, uint32_t tile,
                                                   uint32_t epoch, W &v) {
-    const uint64_t w0 = atomic_load_u64(st + 2 * static_cast<uint64_t>(tile));
-    const uint64_t w1 = atomic_load_u64(st + 2 * static_cast<uint64_t>(tile) + 1);
+    const uint64_t *slot = st + static_cast<uint64_t>(tile) * status_stride_words;
+    const uint64_t w0 = atomic_load_u64(slot);
+    const uint64_t w1 = atomic_load_u64(slot + 1);
     const uint32_t t0 = static_cast<uint32_t>(w0 >> 32);
     const uint32_t t1 = static_cast<uint32_t>(w1 >> 32);
     // Both halves must come from the same publication.
@@ -126,7 +137,6 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
       const uint64_t need = (p >= 63) ? ~0ull : ((2ull << p) - 1);
       const uint64_t bad = wave::ballot(flag == st_invalid) & need;
       if (bad == 0) break;
-      __builtin_amdgcn_s_sleep(1);
     }
     const W x = (lane <= p) ? val : Op::identity();
     const W window = wave::reduce_full<Op>(x);
@@ -212,7 +222,7 @@ __device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoc
 
 // Launch variants (acpp_launch_config::flags).
 enum : uint32_t {
-  variant_wave_lookback = 1u,  // look back with one wave (64-tile window)
+  variant_block_lookback = 1u,  // look back with the whole work-group
 };
 
 template <class T>
@@ -225,7 +235,7 @@ struct params {
   int has_init;
   int exclusive;
   int vec_io;
-  int wave_lookback;
+  int block_lookback;
   uint32_t num_tiles;
   uint32_t epoch;        // tag of this launch's status words (>= 1)
   uint32_t ticket_base;  // tile ticket counter value when this launch starts
@@ -330,7 +340,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
       SIO::publish(p.status, 0, epoch, st_prefix, Op::apply(tile_prefix, agg));
   } else {
     if (tid == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
-    if (p.wave_lookback) {
+    if (!p.block_lookback) {
       if (w == 0) {
         const W excl = lookback<Op, W>(p.status, tile, epoch, &p.sync->watchdog_trips);
         if (lane == 0) {

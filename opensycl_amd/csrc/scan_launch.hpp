@@ -24,7 +24,16 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   acpp_status st = enter_device(g, &cus);
   if (st != ACPP_SUCCESS) return st;
 
-  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 256;
+  // Default work-group size: 1024-wide tiles (fewest tiles, so the fewest
+  // look-back links) when the input still gives every CU a few tiles,
+  // otherwise 256-wide tiles for parallelism on small inputs.
+  uint32_t wg = 256;
+  if (cfg && cfg->work_group_size) {
+    wg = cfg->work_group_size;
+  } else {
+    const int rows0 = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows<T>();
+    if (ceil_div(n, static_cast<uint64_t>(1024) * rows0 * VEC) >= 4ull * static_cast<uint64_t>(cus)) wg = 1024;
+  }
   if (!valid_work_group_size(wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
@@ -44,7 +53,8 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": problem too large");
 
   using SIO = status_io<W>;
-  const uint64_t status_bytes = num_tiles * SIO::words * sizeof(uint64_t);
+  static_assert(SIO::words <= status_stride_words, "status slot too small");
+  const uint64_t status_bytes = num_tiles * status_stride_words * sizeof(uint64_t);
   void *region = nullptr;
   size_t cap = 0;
   region_state *rs = nullptr;
@@ -82,7 +92,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   p.has_init = has_init ? 1 : 0;
   p.exclusive = exclusive ? 1 : 0;
   p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
-  p.wave_lookback = (cfg && (cfg->flags & variant_wave_lookback)) ? 1 : 0;
+  p.block_lookback = (cfg && (cfg->flags & variant_block_lookback)) ? 1 : 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.epoch = epoch;
   p.ticket_base = ticket_base;

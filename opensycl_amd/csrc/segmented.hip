@@ -52,17 +52,21 @@ __device__ __forceinline__ void st64(uint64_t *p, uint64_t v) {
 template <class W, int Bytes = sizeof(W)>
 struct seg_status;
 
+// Each tile's status has a 128-byte line of its own, so that the polls of
+// all tiles in flight do not pile onto a few lines.
+constexpr uint64_t seg_stride_words = 16;
+
 template <class W>
 struct seg_status<W, 4> {
   static constexpr int words = 1;
   __device__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t status,
                                  uint32_t f, W v) {
     const uint64_t tag = (static_cast<uint64_t>(epoch) << 3) | (f << 2) | status;
-    st64(st + tile, (tag << 32) | __builtin_bit_cast(uint32_t, v));
+    st64(st + tile * seg_stride_words, (tag << 32) | __builtin_bit_cast(uint32_t, v));
   }
   __device__ static uint32_t read(const uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t &f,
                                   W &v) {
-    const uint64_t w = ld64(st + tile);
+    const uint64_t w = ld64(st + tile * seg_stride_words);
     const uint32_t tag = static_cast<uint32_t>(w >> 32);
     if ((tag >> 3) != epoch) return st_invalid;
     f = (tag >> 2) & 1u;
@@ -78,13 +82,15 @@ struct seg_status<W, 8> {
                                  uint32_t f, W v) {
     const uint64_t tag = (static_cast<uint64_t>(epoch) << 3) | (f << 2) | status;
     const uint64_t bits = __builtin_bit_cast(uint64_t, v);
-    st64(st + 2 * static_cast<uint64_t>(tile), (tag << 32) | (bits & 0xffffffffu));
-    st64(st + 2 * static_cast<uint64_t>(tile) + 1, (tag << 32) | (bits >> 32));
+    uint64_t *slot = st + tile * seg_stride_words;
+    st64(slot, (tag << 32) | (bits & 0xffffffffu));
+    st64(slot + 1, (tag << 32) | (bits >> 32));
   }
   __device__ static uint32_t read(const uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t &f,
                                   W &v) {
-    const uint64_t w0 = ld64(st + 2 * static_cast<uint64_t>(tile));
-    const uint64_t w1 = ld64(st + 2 * static_cast<uint64_t>(tile) + 1);
+    const uint64_t *slot = st + tile * seg_stride_words;
+    const uint64_t w0 = ld64(slot);
+    const uint64_t w1 = ld64(slot + 1);
     const uint32_t t0 = static_cast<uint32_t>(w0 >> 32);
     const uint32_t t1 = static_cast<uint32_t>(w1 >> 32);
     if (t0 != t1 || (t0 >> 3) != epoch) return st_invalid;
@@ -534,7 +540,8 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   if (st != ACPP_SUCCESS) return st;
   lp.tile_seg = static_cast<uint32_t *>(ts);
   using SIO = seg_status<typename elem_traits<T>::work>;
-  lp.region_used = sizeof(sync_header) + num_tiles * SIO::words * sizeof(uint64_t);
+  static_assert(SIO::words <= seg_stride_words, "status slot too small");
+  lp.region_used = sizeof(sync_header) + num_tiles * seg_stride_words * sizeof(uint64_t);
   st = group_obtain_sync(g, lp.region_used, stream, &lp.region, &lp.region_cap, &lp.rs);
   if (st != ACPP_SUCCESS) return st;
 

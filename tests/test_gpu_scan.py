@@ -63,8 +63,9 @@ def test_pstl_ladder_closed_forms(gpu, q, scratch):
         assert me[0] == 3 and np.all(me[1:] == 0)
 
 
+# 4th field 1 = work-group-wide look-back window instead of one wave
 @pytest.mark.parametrize("cfg", [(64, 2), (128, 4), (256, 2), (256, 4), (512, 8), (1024, 8),
-                                 (1024, 2)])
+                                 (1024, 2), (64, 2, 0, 1), (256, 8, 0, 1), (1024, 8, 0, 1)])
 def test_scan_launch_configs(gpu, q, scratch, cfg):
     torch, _ = gpu
     for dtype in (np.int32, np.int64, np.float32):
