@@ -83,17 +83,22 @@ typedef struct acpp_launch_config {
   uint32_t items_per_thread; /* reduce: 16-byte loads in flight per thread
                                 (2, 4, 8, or 16 except for dot products);
                                 scans: 16-byte rows per thread (2, 4 or 8) */
-  uint32_t groups_per_cu;    /* reduce only: resident work-groups per CU */
+  uint32_t groups_per_cu;    /* reduce and segmented: resident work-groups
+                                per CU (cap on the grid) */
   uint32_t flags;            /* variant bits, 0 = default:
                                 ACPP_VARIANT_BLOCK_LOOKBACK (scans) looks back
                                 with the whole work-group (window = group
                                 size) instead of one wave (window 64);
                                 ACPP_VARIANT_NONTEMPORAL (reduce) streams the
-                                input with non-temporal loads */
+                                input with non-temporal loads;
+                                ACPP_VARIANT_TILE_PER_GROUP (segmented) one
+                                work-group per tile instead of a persistent
+                                grid */
 } acpp_launch_config;
 
 #define ACPP_VARIANT_BLOCK_LOOKBACK 1u
 #define ACPP_VARIANT_NONTEMPORAL 2u
+#define ACPP_VARIANT_TILE_PER_GROUP 4u
 
 /* ------------------------------------------------------------------------ */
 /* Scratch memory: allocation_cache / allocation_group                      */

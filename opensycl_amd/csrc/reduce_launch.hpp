@@ -24,11 +24,24 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   acpp_status st = enter_device(g, &cus);
   if (st != ACPP_SUCCESS) return st;
 
-  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 256;
+  // Library defaults.  For sums and dot products (the tuned path) they are
+  // the configurations measured fastest on MI355X at 2^28 elements: one
+  // resident work-group per CU (512 work-items x 4 loads in flight for a
+  // single operand, 1024 x 8 for a dot product) streaming with
+  // non-temporal loads once the input is larger than the 256 MiB
+  // last-level cache.  The non-temporal choice is the library's only when
+  // the whole config is left at zero.
+  constexpr bool tuned = (Op::id == ops::op_plus);
+  const bool cfg_given = cfg && (cfg->work_group_size || cfg->items_per_thread || cfg->groups_per_cu ||
+                                 cfg->flags);
+  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : (tuned ? (DOT ? 1024 : 512) : 256);
   if (!valid_work_group_size(wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
-  const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : 4;
+  const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : (tuned && DOT ? 8 : 4);
+  const uint32_t groups_per_cu = (cfg && cfg->groups_per_cu) ? cfg->groups_per_cu : (tuned ? 1 : 0);
+  const uint64_t stream_bytes = n * sizeof(typename elem_traits<T>::storage) * (DOT ? 2 : 1);
+  const bool nt = cfg_given ? (cfg->flags & ACPP_VARIANT_NONTEMPORAL) != 0 : (stream_bytes >= (256ull << 20));
   if (unroll != 2 && unroll != 4 && unroll != 8 && (unroll != 16 || DOT))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": items_per_thread must be 2, 4, 8 or 16 (16 only "
@@ -61,8 +74,8 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     acpp_status s = occupancy(kernel, static_cast<int>(wg), &resident);
     if (s != ACPP_SUCCESS) return s;
     uint64_t max_groups = static_cast<uint64_t>(resident) * cus;
-    if (cfg && cfg->groups_per_cu) {
-      uint64_t req = static_cast<uint64_t>(cfg->groups_per_cu) * cus;
+    if (groups_per_cu) {
+      uint64_t req = static_cast<uint64_t>(groups_per_cu) * cus;
       if (req < max_groups) max_groups = req;
     }
     uint64_t needed = ceil_div(nvec + nscalar, static_cast<uint64_t>(wg) * unroll);
@@ -85,8 +98,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   // Only sums (the tuned hot path) get every unroll depth and the
   // non-temporal variant; other operators use 4 to bound the number of
   // kernel instantiations.
-  const bool nt = cfg && (cfg->flags & ACPP_VARIANT_NONTEMPORAL);
-  if constexpr (Op::id == ops::op_plus) {
+  if constexpr (tuned) {
     if (nt) {
       if (unroll == 2) return run(reduce_kernel<T, M, Op, OutT, 2, DOT, true>);
       if (unroll == 8) return run(reduce_kernel<T, M, Op, OutT, 8, DOT, true>);

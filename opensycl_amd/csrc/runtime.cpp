@@ -33,6 +33,10 @@ acpp_status clear_error() {
 acpp_status hip_error(hipError_t e, const char *what) {
   std::string msg = std::string(what) + ": " + hipGetErrorName(e) + " (" +
                     hipGetErrorString(e) + ")";
+  // The error is reported through the returned status; clear HIP's
+  // per-thread "last error" so that the caller's next unrelated HIP call
+  // does not inherit it.
+  (void)hipGetLastError();
   if (e == hipErrorOutOfMemory) return set_error(ACPP_ERROR_OUT_OF_MEMORY, msg);
   return set_error(ACPP_ERROR_HIP, msg);
 }

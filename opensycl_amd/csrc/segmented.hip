@@ -126,6 +126,56 @@ __device__ __forceinline__ void wave_pair_scan(uint32_t &f, W &v) {
 #undef ACPP_SEG_STEP
 }
 
+// Ordered look-back by one wave: lane l polls tile (wbase - 63 + l), lane 63
+// the nearest predecessor.  It stops at the nearest tile that is a prefix
+// or an aggregate containing a segment start, and folds the window from
+// there in tile order.  Must be called by a whole wave; result uniform.
+template <class Op, class W>
+__device__ void wave_pair_lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
+                                   uint32_t *watchdog, uint32_t &out_f, W &out_v) {
+  using SIO = seg_status<W>;
+  const int lane = threadIdx.x & 63;
+  const W id = Op::identity();
+  int64_t wbase = static_cast<int64_t>(tile) - 1;
+  uint32_t ef = 0;  // accumulated prefix of tiles nearer than the window
+  W ev = id;
+  uint32_t polls = 0;
+  for (;;) {
+    const int64_t j = wbase - (63 - lane);
+    uint32_t lf = 0;
+    W lv = id;
+    int q;
+    for (;;) {
+      uint32_t st = st_prefix;  // positions before tile 0 act as an empty prefix
+      lf = 0;
+      lv = id;
+      if (j >= 0) st = SIO::read(status, static_cast<uint32_t>(j), epoch, lf, lv);
+      if (++polls > lookback_poll_limit) {
+        if (lane == 0 && polls == lookback_poll_limit + 1)
+          __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        st = st_prefix;
+      }
+      const bool term = (st == st_prefix) || (st == st_aggregate && lf);
+      const uint64_t tm = wave::ballot(term);
+      q = tm ? 63 - __builtin_clzll(tm) : -1;
+      const uint64_t need = (q <= 0) ? ~0ull : (~0ull << q);  // lanes q..63
+      const uint64_t bad = wave::ballot(st == st_invalid) & need;
+      if (bad == 0) break;
+    }
+    uint32_t xf = (lane >= q) ? lf : 0u;
+    W xv = (lane >= q) ? lv : id;
+    wave_pair_scan<Op>(xf, xv);
+    uint32_t wf = wave::read_lane(xf, 63);
+    W wv = wave::read_lane(xv, 63);
+    // window (earlier) . accumulated (later)
+    combine<Op>(wf, wv, ef, ev);
+    if (q >= 0) break;
+    wbase -= 64;
+  }
+  out_f = ef;
+  out_v = ev;
+}
+
 template <class T>
 struct params {
   using W = typename elem_traits<T>::work;
@@ -133,6 +183,9 @@ struct params {
   T *out;
   const uint64_t *offsets;  // num_segments + 1 entries
   const uint32_t *tile_seg;  // first segment of every tile (num_tiles + 1 entries)
+  const uint32_t *heads;     // bit e set: a non-empty segment starts at element e
+  uint64_t head_words_total;  // 32-bit words of the bitmap
+  int block_lookback;        // 1: work-group-wide look-back window
   uint64_t num_segments;
   uint64_t total;
   W init;
@@ -151,9 +204,11 @@ __device__ __forceinline__ uint64_t clamp_off(const uint64_t *offsets, uint64_t 
 }
 
 // Setup: tile_seg[t] = the non-empty segment containing element t*tile_elems
-// (num_segments for t == num_tiles), and init into empty segments' results.
+// (num_segments for t == num_tiles), the head bitmap (bit b for the first
+// element b of every non-empty segment; the bitmap is cleared beforehand),
+// and init into empty segments' results.
 template <class T>
-__global__ void setup_kernel(params<T> p, uint32_t *tile_seg_out, uint64_t tile_elems) {
+__global__ void setup_kernel(params<T> p, uint32_t *tile_seg_out, uint32_t *heads_out, uint64_t tile_elems) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < p.num_segments;
        s += stride) {
@@ -163,6 +218,7 @@ __global__ void setup_kernel(params<T> p, uint32_t *tile_seg_out, uint64_t tile_
       if (p.reduce) p.out[s] = static_cast<T>(p.init);
       continue;
     }
+    atomicOr(&heads_out[b >> 5], 1u << (b & 31));
     // tiles whose first element lies in [b, e)
     const uint64_t t0 = (b + tile_elems - 1) / tile_elems;
     const uint64_t t1 = (e + tile_elems - 1) / tile_elems;
@@ -207,59 +263,47 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
   __shared__ uint64_t s_invalid[max_waves];
   const uint32_t head_words = static_cast<uint32_t>((tile_elems + 1 + 31) / 32);
 
+  // A ticket is drawn only when the work-group is ready to work on the
+  // tile: every later tile's look-back waits for this tile's aggregate, so a
+  // ticket held while the group is busy with another tile stalls the whole
+  // chain (drawing one tile ahead made the 65,536-segment int64 scan ~3x
+  // slower on MI355X).
   for (;;) {
     __syncthreads();  // LDS of the previous tile is free again
     if (tid == 0)
       s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT) -
                p.ticket_base;
-    for (uint32_t i = tid; i < head_words; i += B) s_heads[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= p.num_tiles) return;  // queue drained
 
     const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
-    const uint64_t tile_end = (base + tile_elems < p.total) ? base + tile_elems : p.total;
-    // Segments touching the tile: [s_lo, s_hi].  Indices are clamped so that
-    // malformed offsets can give wrong results but never out-of-bounds
-    // accesses.
+    // Head bits of elements [base, base + tile_elems] (base is a multiple
+    // of 32).  Loaded before the tile's data so that waiting for them does
+    // not wait for the data (loads retire in order).
+    uint32_t hword0 = 0, hword1 = 0;
+    {
+      const uint64_t i0 = (base >> 5) + tid;
+      const uint64_t i1 = i0 + B;
+      if (static_cast<uint32_t>(tid) < head_words && i0 < p.head_words_total) hword0 = p.heads[i0];
+      if (static_cast<uint32_t>(tid + B) < head_words && i1 < p.head_words_total) hword1 = p.heads[i1];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // Segments touching the tile: [s_lo, s_hi] (used to number the outputs
+    // of a reduction).  Indices are clamped so that malformed offsets can
+    // give wrong results but never out-of-bounds accesses.
     uint64_t s_lo = p.tile_seg[tile];
     uint64_t s_hi = p.tile_seg[tile + 1];
-    if (s_lo >= p.num_segments) s_lo = p.num_segments - 1;
-    if (s_hi >= p.num_segments) s_hi = p.num_segments - 1;
-    if (s_hi < s_lo) s_hi = s_lo;
-    // Mark segment starts in [base, tile_end] (bit tile_elems: a segment
-    // starts exactly at the next tile).
-    for (uint64_t s = s_lo + tid; s <= s_hi; s += B) {
-      const uint64_t o = clamp_off(p.offsets, s, p.total);
-      if (o >= base && o <= tile_end && o < p.total) {
-        const uint64_t pos = o - base;
-        atomicOr(&s_heads[pos >> 5], 1u << (pos & 31));
-      }
-    }
-    __syncthreads();
-
-    auto head_at = [&](uint64_t pos) -> uint32_t {
-      return (s_heads[pos >> 5] >> (pos & 31)) & 1u;
-    };
 
     // Load the tile.
     const bool full = p.vec_io && (base + tile_elems <= p.total);
     W v[R][VEC];
-    // fm[r] bit k: element k of row r starts a segment.
-    uint32_t fm[R];
+    u32x4 q[R];
     if (full) {
       const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
-      u32x4 q[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        T tmp[VEC];
-        __builtin_memcpy(tmp, &q[r], 16);
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
-      }
     } else {
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -269,6 +313,28 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
           v[r][k] = (idx < p.total) ? elem_traits<T>::to_work(p.in[idx]) : id;
         }
     }
+    if (static_cast<uint32_t>(tid) < head_words) s_heads[tid] = hword0;
+    if (static_cast<uint32_t>(tid + B) < head_words) s_heads[tid + B] = hword1;
+    if (s_lo >= p.num_segments) s_lo = p.num_segments - 1;
+    if (s_hi >= p.num_segments) s_hi = p.num_segments - 1;
+    if (s_hi < s_lo) s_hi = s_lo;
+    __syncthreads();
+    if (full) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T tmp[VEC];
+        __builtin_memcpy(tmp, &q[r], 16);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+      }
+    }
+
+    auto head_at = [&](uint64_t pos) -> uint32_t {
+      return (s_heads[pos >> 5] >> (pos & 31)) & 1u;
+    };
+
+    // fm[r] bit k: element k of row r starts a segment.
+    uint32_t fm[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       // VEC consecutive bits of the bitmap (VEC divides 32, so they never
@@ -325,7 +391,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
     const uint32_t agg_f = cf;
     const W agg_v = cv;
 
-    // Tile prefix: ordered block-wide look-back.
+    // Tile prefix: ordered look-back (one wave, or the whole work-group).
     uint32_t pf = 0;
     W pv = id;
     if (tile == 0) {
@@ -336,7 +402,27 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
       if (tid == 0) SIO::publish(p.status, tile, p.epoch, agg_f ? st_prefix : st_aggregate, agg_f, agg_v);
       // Only elements before the tile's first segment start need the
       // prefix; a tile that begins with a segment start needs none.
-      if (!head_at(0)) {
+      if (!head_at(0) && !p.block_lookback) {
+        // One wave looks back; its first lane publishes the tile's prefix.
+        if (w == 0) {
+          uint32_t ef;
+          W ev;
+          wave_pair_lookback<Op, W>(p.status, tile, p.epoch, &p.sync->watchdog_trips, ef, ev);
+          if (lane == 0) {
+            s_lbf[0] = ef;
+            s_lbv[0] = ev;
+            if (!agg_f && tile + 1 < p.num_tiles) {
+              uint32_t of = agg_f;
+              W ov = agg_v;
+              combine<Op>(ef, ev, of, ov);
+              SIO::publish(p.status, tile, p.epoch, st_prefix, of, ov);
+            }
+          }
+        }
+        __syncthreads();
+        pf = s_lbf[0];
+        pv = s_lbv[0];
+      } else if (!head_at(0)) {
         // thread t polls tile (base_t - (B - 1 - t)); thread B-1 = nearest.
         int64_t wbase = static_cast<int64_t>(tile) - 1;
         uint32_t ef = 0;  // accumulated prefix of tiles nearer than the window
@@ -481,12 +567,15 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
 // --------------------------------------------------------------------------
 // Host side
 // --------------------------------------------------------------------------
-// Four 16-byte rows per work-item keep every variant below 128 VGPRs; eight
-// rows spill and are only used when a caller asks for them.
+// Default tile: 512 work-items x four 16-byte rows, the fastest on MI355X
+// for large inputs (65,536 log-normal segments; and every shape above 2^24
+// elements in the tuner's timings).  Eight rows spill and are only used
+// when a caller asks for them.
 template <class T>
 int default_rows() {
   return 4;
 }
+constexpr uint32_t default_work_group = 512;
 
 template <class T, class Op>
 struct launch_plan {
@@ -497,6 +586,8 @@ struct launch_plan {
   size_t lds = 0;
   uint64_t tile_elems = 0;
   uint32_t *tile_seg = nullptr;
+  uint32_t *heads = nullptr;
+  uint64_t head_words = 0;
   void *region = nullptr;
   region_state *rs = nullptr;
   size_t region_used = 0;
@@ -505,9 +596,11 @@ struct launch_plan {
 
 template <class T, class Op, int R>
 acpp_status launch_rows(launch_plan<T, Op> &lp, hipStream_t stream, const char *name) {
+  acpp_status st = fill_zero(lp.heads, lp.head_words * sizeof(uint32_t), stream);
+  if (st != ACPP_SUCCESS) return st;
   hipLaunchKernelGGL(setup_kernel<T>, dim3(static_cast<uint32_t>(std::min<uint64_t>(ceil_div(lp.p.num_segments + 1, 256), 4096))),
-                     dim3(256), 0, stream, lp.p, lp.tile_seg, lp.tile_elems);
-  acpp_status st = check_launch(name);
+                     dim3(256), 0, stream, lp.p, lp.tile_seg, lp.heads, lp.tile_elems);
+  st = check_launch(name);
   if (st != ACPP_SUCCESS) return st;
   hipLaunchKernelGGL((segmented_kernel<T, Op, R>), dim3(lp.grid), dim3(lp.wg), lp.lds, stream, lp.p);
   return check_launch(name);
@@ -523,7 +616,7 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   acpp_status st = enter_device(g, &cus);
   if (st != ACPP_SUCCESS) return st;
   launch_plan<T, Op> lp;
-  lp.wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 256;
+  lp.wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : default_work_group;
   if (!valid_work_group_size(lp.wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad work_group_size");
   lp.rows = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows<T>();
@@ -539,6 +632,12 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   st = group_obtain(g, (num_tiles + 1) * sizeof(uint32_t), &ts);
   if (st != ACPP_SUCCESS) return st;
   lp.tile_seg = static_cast<uint32_t *>(ts);
+  // Head bitmap over elements [0, total] plus one spare word.
+  lp.head_words = (total + 1 + 31) / 32 + 1;
+  void *hb = nullptr;
+  st = group_obtain(g, lp.head_words * sizeof(uint32_t), &hb);
+  if (st != ACPP_SUCCESS) return st;
+  lp.heads = static_cast<uint32_t *>(hb);
   using SIO = seg_status<typename elem_traits<T>::work>;
   static_assert(SIO::words <= seg_stride_words, "status slot too small");
   lp.region_used = sizeof(sync_header) + num_tiles * seg_stride_words * sizeof(uint64_t);
@@ -568,6 +667,9 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   p.out = out;
   p.offsets = offsets;
   p.tile_seg = lp.tile_seg;
+  p.heads = lp.heads;
+  p.head_words_total = lp.head_words;
+  p.block_lookback = (cfg && (cfg->flags & ACPP_VARIANT_BLOCK_LOOKBACK)) ? 1 : 0;
   p.num_segments = num_segments;
   p.total = total;
   p.init = init;
@@ -579,10 +681,18 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   p.sync = static_cast<sync_header *>(lp.region);
   p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(lp.region) + sizeof(sync_header));
 
-  // Persistent grid: one wave of resident work-groups (each draws tiles
-  // until the queue is empty).  Extra tickets drawn at the end are counted
-  // in the ticket base.
+  // Persistent grid: one wave of resident work-groups, each drawing tiles
+  // from the queue until it is empty.  With ACPP_VARIANT_TILE_PER_GROUP the
+  // grid has one work-group per tile (the hardware dispatcher then keeps
+  // the CUs full while groups wait in their look-back), still fed by the
+  // same queue.  Every work-group draws one ticket past the last tile
+  // before it exits; the ticket base counts those too.
+  const bool per_tile = cfg && (cfg->flags & ACPP_VARIANT_TILE_PER_GROUP);
   auto pick = [&](auto kernel) -> acpp_status {
+    if (per_tile) {
+      lp.grid = static_cast<uint32_t>(num_tiles);
+      return ACPP_SUCCESS;
+    }
     int resident = 1;
     ACPP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, static_cast<int>(lp.wg), lp.lds));
     if (resident < 1) resident = 1;
@@ -663,7 +773,10 @@ acpp_status capture_plan(acpp_allocation_group *g, F &&enqueue_fn, acpp_segmente
   ACPP_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   std::unique_ptr<acpp_segmented_plan> plan(new acpp_segmented_plan);
   plan->device = group_device(g);
-  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  // Relaxed mode: the first capture of a plan may have to allocate its
+  // scratch (hipMalloc through the allocation cache); the allocation is made
+  // at once and is not part of the graph.
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) {
     (void)hipStreamDestroy(cs);
     return hip_error(e, "hipStreamBeginCapture");

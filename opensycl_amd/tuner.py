@@ -46,7 +46,7 @@ WORK_GROUP_SIZES = (64, 128, 256, 512, 1024)
 ROWS = (2, 4, 8)
 CONFIGS = [(wg, r) for wg in WORK_GROUP_SIZES for r in ROWS]
 # The library's own choice when no configuration is given (segmented.hip).
-DEFAULT_CONFIG = (256, 4)
+DEFAULT_CONFIG = (512, 4)
 
 KINDS = ("scan", "reduce")
 DTYPES = {"i32": 4, "i64": 8, "f32": 4, "f64": 8}
@@ -71,8 +71,28 @@ class Shape:
         return np.concatenate([[0], np.cumsum(np.asarray(self.lengths, dtype=np.uint64))]).astype(np.uint64)
 
 
-def shape_features(shape: Shape) -> np.ndarray:
+@dataclass
+class ShapeSummary:
+    """What the timing database keeps of a shape: its type, operation, sizes
+    and the 11 shape features (the segment lengths themselves are not
+    needed by the cost model)."""
+    kind: str
+    dtype: str
+    op: str
+    n: int
+    num_segments: int
+    stats: list
+
+
+def summarize(shape: Shape) -> ShapeSummary:
+    return ShapeSummary(shape.kind, shape.dtype, shape.op, shape.n, shape.num_segments,
+                        [float(v) for v in shape_features(shape)])
+
+
+def shape_features(shape) -> np.ndarray:
     """The 11 shape features (element count, segment statistics, type, op)."""
+    if isinstance(shape, ShapeSummary):
+        return np.asarray(shape.stats, dtype=np.float64)
     lens = np.asarray(shape.lengths, dtype=np.float64)
     ll = np.log2(lens + 1.0)
     mu = float(ll.mean()) if len(ll) else 0.0
@@ -93,7 +113,7 @@ def shape_features(shape: Shape) -> np.ndarray:
     ])
 
 
-def config_features(shape: Shape, cfg) -> np.ndarray:
+def config_features(shape, cfg) -> np.ndarray:
     """The 5 configuration features (geometry relative to the shape)."""
     wg, rows = cfg
     tile = wg * rows * (16 // DTYPES[shape.dtype])
@@ -107,7 +127,7 @@ def config_features(shape: Shape, cfg) -> np.ndarray:
     ])
 
 
-def features(shape: Shape, cfg) -> np.ndarray:
+def features(shape, cfg) -> np.ndarray:
     f = np.concatenate([shape_features(shape), config_features(shape, cfg)])
     assert f.shape == (MLP_INPUTS,)
     return f
@@ -144,8 +164,6 @@ def lognormal_workload(kind: str, dtype: str, S: int = 65536, seed: int = 0) -> 
 # ---------------------------------------------------------------------------
 # Timing on the GPU
 # ---------------------------------------------------------------------------
-_TORCH_DTYPE = None
-
 
 def _torch_dtype(tag):
     import torch
@@ -233,7 +251,7 @@ def collect(num_shapes: int, seed: int = 0, reps: int = 5, path: Optional[str] =
     for i in range(num_shapes):
         shape = sample_shape(rng)
         times = autotune(shape, reps=reps, device=device, seed=seed * 100003 + i)
-        db.append({"shape": asdict(shape), "times_us": {_key(c): t for c, t in times.items()}})
+        db.append({"shape": asdict(summarize(shape)), "times_us": {_key(c): t for c, t in times.items()}})
         if progress and (i + 1) % 10 == 0:
             print(f"[tuner] {i + 1}/{num_shapes} shapes timed ({time.time() - t0:.0f} s)", flush=True)
         if path and ((i + 1) % 50 == 0 or i + 1 == num_shapes):
@@ -253,8 +271,8 @@ def load_db(path: str) -> list:
         return json.load(f)
 
 
-def record_shape(rec: dict) -> Shape:
-    return Shape(**rec["shape"])
+def record_shape(rec: dict) -> ShapeSummary:
+    return ShapeSummary(**rec["shape"])
 
 
 def training_set(db: list):

@@ -83,7 +83,10 @@ def test_segmented_reduce(gpu, q, scratch, dtype):
                 assert np.array_equal(got, O.segmented_reduce(x, off, init, op)), (name, op)
 
 
-@pytest.mark.parametrize("cfg", [(64, 2), (128, 4), (256, 2), (512, 4), (1024, 2), (1024, 4)])
+# flags: 1 = work-group-wide look-back window, 4 = one work-group per tile
+@pytest.mark.parametrize("cfg", [(64, 2), (128, 4), (256, 2), (512, 4), (1024, 2), (1024, 4),
+                                 (256, 4, 0, 1), (256, 4, 0, 4), (512, 8, 0, 5), (64, 2, 0, 4),
+                                 (1024, 8, 0, 0)])
 def test_segmented_configs(gpu, q, scratch, cfg):
     torch, _ = gpu
     off = lognormal_offsets(2000, seed=3, max_len=1 << 17)
@@ -152,3 +155,25 @@ def test_segmented_lognormal_workload_shape(gpu, q, scratch):
     y = host(yd)
     del yd
     assert np.array_equal(y, O.segmented_inclusive_scan(x, off))
+
+
+def test_segmented_plan_first_capture_allocates(gpu, q):
+    # A plan made with a fresh cache has to allocate its scratch while the
+    # launch is being captured; the plan must still work, and later torch
+    # work must not see a stale HIP error.
+    torch, _ = gpu
+    off = lognormal_offsets(500, seed=8, max_len=1 << 15)
+    n = int(off[-1])
+    x = rand(np.int64, n, seed=4, lo=-1000, hi=1000)
+    xd, od = dev(torch, x), dev(torch, off)
+    yd = torch.zeros_like(xd)
+    cache = acpp.AllocationCache()
+    with acpp.AllocationGroup(cache, 0) as g:
+        plan = acpp.SegmentedPlan.inclusive_scan(g, xd, yd, od)
+        plan.launch(q)
+        q.wait()
+        assert np.array_equal(host(yd), O.segmented_inclusive_scan(x, off))
+        z = torch.ones(1000, device="cuda").cumsum(0)  # torch work after the capture
+        assert z[-1].item() == 1000.0
+        plan.close()
+    cache.close()

@@ -51,9 +51,10 @@ def test_lognormal_workload():
 
 
 def test_regret_bookkeeping():
-    db = [{"shape": {}, "times_us": {"256x4": 10.0, "512x8": 5.0, "64x2": 20.0}},
-          {"shape": {}, "times_us": {"256x4": 8.0, "512x8": 8.0, "64x2": 4.0}}]
-    r = T.regret(db, [(512, 8), (256, 4)])
+    d = "%dx%d" % T.DEFAULT_CONFIG
+    db = [{"shape": {}, "times_us": {d: 10.0, "1024x8": 5.0, "64x2": 20.0}},
+          {"shape": {}, "times_us": {d: 8.0, "1024x8": 8.0, "64x2": 4.0}}]
+    r = T.regret(db, [(1024, 8), T.DEFAULT_CONFIG])
     assert r["mean_regret"] == pytest.approx((0.0 + 1.0) / 2)
     assert r["default_mean_regret"] == pytest.approx((1.0 + 1.0) / 2)
     assert r["geomean_speedup_vs_default"] == pytest.approx(math.sqrt(2.0 * 1.0))
