@@ -26,7 +26,8 @@ Other fields:
                    with 1% of entries scaled by 1e4 (relative error against
                    a float64 torch sum), histograms of Zipf(1.1) keys into
                    4096 and 65536 bins and fetch_max into 1024 slots, with
-                   LDS-privatised and direct global atomics;
+                   LDS-privatised and direct global atomics, and copy_if of
+                   2^28 int32 keeping the even values;
   cpu_baseline  -- the oracle's OpenMP restatement of the reference's
                    OpenMP-backend algorithms on this host's cores, on the
                    headline workload, with the GPU results checked against it.
@@ -428,6 +429,25 @@ def extras_leg(acpp, q, cache, log2n):
             total = int(hist["privatized"].to(torch.int64).sum().item())
             res[f"histogram_{bins}_check"] = {"strategies_bit_identical": same, "sum_equals_n": total == n}
             del keys
+        # copy_if (stream compaction) of 2^28 int32 U[-2^15, 2^15] with
+        # (x & 1) == 0: about half the elements selected
+        xi = torch.randint(-(1 << 15), (1 << 15) + 1, (n,), dtype=torch.int32, device="cuda", generator=gen)
+        di = torch.empty_like(xi)
+        cnt64 = torch.zeros(1, dtype=torch.int64, device="cuda")
+
+        def runc():
+            acpp.copy_if(q, g, xi, di, acpp.PRED_MASK_EQ, 0, 1, cnt64.view(torch.uint64))
+        runc()
+        g.release()
+        avg, _ = timed(q, runc, 10, release=g.release)
+        c = int(cnt64.item())
+        sel = (xi & 1) == 0
+        res["copy_if_i32_even"] = {
+            "avg_us": round(avg, 2), "selected": c, "algorithmic_bytes": 4 * n + 4 * c,
+            "gbs": round((4 * n + 4 * c) / (avg * 1e-6) / 1e9, 1),
+            "count_matches_torch": c == int(sel.sum().item()),
+            "output_matches_torch": bool(torch.equal(di[:c], xi[sel]))}
+        del xi, di, sel
         vals = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32, device="cuda", generator=gen)
         tmax = {}
         for strat_name, strat in (("privatized", acpp.ATOMICS_PRIVATIZED), ("global", acpp.ATOMICS_GLOBAL)):

@@ -314,6 +314,38 @@ acpp_status acpp_segmented_plan_launch(acpp_segmented_plan *plan, acpp_stream_t 
 acpp_status acpp_segmented_plan_destroy(acpp_segmented_plan *plan);
 
 /* ------------------------------------------------------------------------ */
+/* copy_if: stream compaction                                               */
+/* Replaces algorithms::copy_if (include/hipSYCL/algorithms/algorithm.hpp:  */
+/* 178-243): the elements x[i] for which the predicate holds are written,   */
+/* in order, to d_first[0..count); *num_copied = count (device pointer, may */
+/* be NULL; 0 for an empty range, algorithm.hpp:184-188).  One launch:      */
+/* predicate bits, DPP ranks, decoupled look-back over the tile counts.     */
+/* The predicate is given as data (a C functor cannot cross this ABI):      */
+/* ------------------------------------------------------------------------ */
+#define ACPP_PRED_NONZERO 0 /* x != 0 */
+#define ACPP_PRED_EQ 1      /* x == operand */
+#define ACPP_PRED_NE 2      /* x != operand */
+#define ACPP_PRED_LT 3      /* x <  operand */
+#define ACPP_PRED_LE 4      /* x <= operand */
+#define ACPP_PRED_GT 5      /* x >  operand */
+#define ACPP_PRED_GE 6      /* x >= operand */
+#define ACPP_PRED_MASK_EQ 7 /* (x & mask) == operand, integers only */
+#define ACPP_PRED_STENCIL 8 /* stencil[i] != 0 (the _stencil entry points) */
+acpp_status acpp_copy_if_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, int pred, int32_t operand, int32_t mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *d_first, int pred, uint32_t operand, uint32_t mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, int pred, int64_t operand, int64_t mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *d_first, int pred, uint64_t operand, uint64_t mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, int pred, float operand, float mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, int pred, double operand, double mask, uint64_t *num_copied, const acpp_launch_config *config);
+/* Predicate evaluated by the caller: copies x[i] where stencil[i] != 0. */
+acpp_status acpp_copy_if_stencil_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, const uint8_t *stencil, int32_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, const uint8_t *stencil, uint32_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, const uint8_t *stencil, int64_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, const uint8_t *stencil, uint64_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, const uint8_t *stencil, float *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, const uint8_t *stencil, double *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+
+/* ------------------------------------------------------------------------ */
 /* Learned cost model (no counterpart in the reference, whose launch        */
 /* geometry is fixed: numeric.hpp:144-156, decoupled_lookback_scan.hpp:     */
 /* 481-488).  A 16 -> 64 -> 64 -> 1 tanh network that predicts the          */

@@ -4,6 +4,7 @@ The compute path is the HIP library ``lib/libacpp_mi355x.so`` (gfx950),
 reached through the C-ABI in ``include/acpp_mi355x.h``.
 """
 from . import _abi  # noqa: F401
+from ._abi import AcppError  # noqa: F401
 from .algorithms import (ALLOCATION_DEVICE, ALLOCATION_HOST, ALLOCATION_SHARED,  # noqa: F401
                          ATOMICS_AUTO, ATOMICS_GLOBAL, ATOMICS_PRIVATIZED, atomic_fetch_strided,
                          histogram, SegmentedPlan, segmented_inclusive_scan, segmented_reduce,
@@ -11,6 +12,8 @@ from .algorithms import (ALLOCATION_DEVICE, ALLOCATION_HOST, ALLOCATION_SHARED, 
                          PLUS, AllocationCache, AllocationGroup, Queue, compute_units,
                          exclusive_scan, inclusive_scan, reduce, reduce_compensated, reduction,
                          transform_reduce)
+from .algorithms import (PRED_EQ, PRED_GE, PRED_GT, PRED_LE, PRED_LT, PRED_MASK_EQ,  # noqa: F401
+                         PRED_NE, PRED_NONZERO, copy_if, copy_if_stencil)
 from .cost_model import CostModel  # noqa: F401
 
 __version__ = "0.1.0"

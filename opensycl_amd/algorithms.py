@@ -302,6 +302,39 @@ def atomic_fetch_strided(q: Queue, scratch: AllocationGroup, op: str, values, ta
 
 
 # ---------------------------------------------------------------------------
+# copy_if (stream compaction)
+# ---------------------------------------------------------------------------
+PRED_NONZERO, PRED_EQ, PRED_NE, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_MASK_EQ = range(8)
+
+
+def copy_if(q: Queue, scratch: AllocationGroup, x, d, pred: int, operand=0, mask=0, num_copied=None,
+            config=None) -> None:
+    """algorithms::copy_if (algorithm.hpp:178-243) with a data predicate:
+    d[0..count) = the x[i] with pred(x[i]); num_copied[0] = count (a uint64
+    device tensor, optional)."""
+    _check_contig(x, d)
+    if num_copied is not None and _tag(num_copied.dtype) != "u64":
+        raise TypeError("num_copied must be a uint64 tensor")
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_copy_if_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), int(pred), _scalar(tag, operand),
+             _scalar(tag, mask), _ptr(num_copied) if num_copied is not None else None, _cfg(config)))
+
+
+def copy_if_stencil(q: Queue, scratch: AllocationGroup, x, stencil, d, num_copied=None,
+                    config=None) -> None:
+    """copy_if with the predicate evaluated by the caller: copies x[i] where
+    stencil[i] != 0 (stencil: uint8 or bool tensor of x's length)."""
+    _check_contig(x, d, stencil)
+    if stencil.numel() < x.numel() or stencil.element_size() != 1:
+        raise ValueError("stencil must be a one-byte tensor at least as long as x")
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_copy_if_stencil_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(stencil), _ptr(d),
+             _ptr(num_copied) if num_copied is not None else None, _cfg(config)))
+
+
+# ---------------------------------------------------------------------------
 # Segmented scans and reductions
 # ---------------------------------------------------------------------------
 def _seg_check(x, offsets):

@@ -1,0 +1,311 @@
+// Stream compaction: algorithms::copy_if
+// (include/hipSYCL/algorithms/algorithm.hpp:178-243).
+//
+// Reference: an exclusive scan (generate_scan_process) of size_t flags
+// pred(x[i]) ? 1 : 0, whose result processor writes x[i] to
+// d_first[scan[i]] and, at the last element, the number of elements copied
+// (algorithm.hpp:194-242).  An empty range writes 0 to *num_elements_copied
+// and returns (algorithm.hpp:184-188).
+//
+// MI355X design: one launch on the decoupled look-back machinery of the scan
+// (scan_common.hpp).  A tile is B work-items x R rows x one 16-byte vector;
+// each work-item evaluates the predicate on its vectors into a bitmask,
+// counts with popcount, the work-group ranks the selected elements with DPP
+// wave scans and one LDS exchange, one wave looks back over the tile
+// counts (64-bit, one 128-byte status line per tile) for the tile's output
+// offset.  Each row's selected elements are compacted in LDS and stored
+// with consecutive work-items on consecutive output elements (coalesced
+// whatever the selection pattern).  The last tile writes the total count.
+//
+// The predicate is data, not a functor (a functor cannot cross the C-ABI):
+// comparison with a constant, a bit-mask test, or a stencil array for any
+// predicate the caller evaluates with its own kernel.
+#include "launch_util.hpp"
+#include "scan_common.hpp"
+
+namespace acpp_impl {
+namespace copy_if_detail {
+
+using scan_detail::st_aggregate;
+using scan_detail::st_prefix;
+using scan_detail::status_stride_words;
+using scan_detail::u32x4;
+
+template <class T>
+struct params {
+  const T *in;
+  const uint8_t *stencil;  // ACPP_PRED_STENCIL
+  T *out;
+  uint64_t *num_copied;     // may be NULL
+  uint64_t n;
+  int pred;
+  T operand;
+  T mask;
+  int vec_io;
+  uint32_t num_tiles;
+  uint32_t epoch;
+  uint32_t ticket_base;
+  sync_header *sync;
+  uint64_t *status;
+};
+
+template <class T>
+__device__ __forceinline__ bool eval(const params<T> &p, T x, uint64_t idx) {
+  switch (p.pred) {
+    case ACPP_PRED_NONZERO: return x != T(0);
+    case ACPP_PRED_EQ: return x == p.operand;
+    case ACPP_PRED_NE: return x != p.operand;
+    case ACPP_PRED_LT: return x < p.operand;
+    case ACPP_PRED_LE: return x <= p.operand;
+    case ACPP_PRED_GT: return x > p.operand;
+    case ACPP_PRED_GE: return x >= p.operand;
+    case ACPP_PRED_MASK_EQ:
+      if constexpr (std::is_integral_v<T>) return (x & p.mask) == p.operand;
+      return false;
+    default: return p.stencil[idx] != 0;  // ACPP_PRED_STENCIL
+  }
+}
+
+template <class T, int R>
+__global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
+  using CountOp = ops::plus<uint32_t, uint32_t>;
+  using SIO = scan_detail::status_io<uint64_t>;
+  constexpr int VEC = 16 / sizeof(T);
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int nw = B >> 6;
+  const int w = tid >> 6;
+  const int lane = tid & 63;
+
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wave[R][scan_detail::max_waves];
+  __shared__ uint64_t s_excl;
+  extern __shared__ unsigned char s_stage_raw[];  // one compacted row: B * VEC elements
+  T *s_stage = reinterpret_cast<T *>(s_stage_raw);
+
+  if (tid == 0)
+    s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+             p.ticket_base;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  if (tile >= p.num_tiles) return;
+
+  const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
+  const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+  const bool full = p.vec_io && (base + tile_elems <= p.n);
+
+  T v[R][VEC];
+  if (full) {
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
+    u32x4 q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+#pragma unroll
+    for (int r = 0; r < R; ++r) __builtin_memcpy(v[r], &q[r], 16);
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
+        v[r][k] = (idx < p.n) ? p.in[idx] : T(0);
+      }
+  }
+  // Selection bits: bit k of fm[r] = element k of row r is copied.
+  uint32_t fm[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    fm[r] = 0;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
+      if (idx < p.n && eval(p, v[r][k], idx)) fm[r] |= 1u << k;
+    }
+  }
+  // Ranks inside the tile.
+  uint32_t wexcl[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t incl = wave::inclusive_scan<CountOp>(static_cast<uint32_t>(__builtin_popcount(fm[r])));
+    wexcl[r] = wave::shift_up<CountOp>(incl);
+    if (lane == 63) s_wave[r][w] = incl;
+  }
+  __syncthreads();
+  uint32_t before[R];
+  uint32_t carry = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    before[r] = 0;
+    for (int ww = 0; ww < nw; ++ww) {
+      const uint32_t x = s_wave[r][ww];
+      if (ww == w) before[r] = carry;
+      carry += x;
+    }
+  }
+  const uint64_t agg = carry;
+
+  // Output offset of the tile.
+  uint64_t tile_prefix = 0;
+  if (tile == 0) {
+    if (tid == 0 && p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, agg);
+  } else {
+    if (w == 0) {
+      if (lane == 0) SIO::publish(p.status, tile, p.epoch, st_aggregate, agg);
+      const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t>(
+          p.status, tile, p.epoch, &p.sync->watchdog_trips);
+      if (lane == 0) {
+        if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, p.epoch, st_prefix, e + agg);
+        s_excl = e;
+      }
+    }
+    __syncthreads();
+    tile_prefix = s_excl;
+  }
+  if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + agg;
+
+  // Row by row: compact the row's selected elements in LDS, then store the
+  // row's output range with consecutive work-items on consecutive elements
+  // (fully coalesced, whatever the selection pattern).
+  uint32_t row_start = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    uint32_t row_count = 0;
+    for (int ww = 0; ww < nw; ++ww) row_count += s_wave[r][ww];
+    uint32_t pos = before[r] - row_start + wexcl[r];
+    const uint32_t bits = fm[r];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      if (bits & (1u << k)) s_stage[pos++] = v[r][k];
+    }
+    __syncthreads();
+    T *dst = p.out + tile_prefix + row_start;
+    for (uint32_t i = tid; i < row_count; i += B) dst[i] = s_stage[i];
+    __syncthreads();  // the stage is rewritten by the next row
+    row_start += row_count;
+  }
+}
+
+__global__ void store_zero_kernel(uint64_t *p) { *p = 0; }
+
+template <class T>
+constexpr int rows() {
+  return 8;
+}
+
+template <class T>
+acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uint64_t n, const uint8_t *stencil,
+                  T *out, int pred, T operand, T mask, uint64_t *num_copied, const acpp_launch_config *cfg,
+                  const char *name) {
+  constexpr int VEC = 16 / sizeof(T);
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (pred < ACPP_PRED_NONZERO || pred > ACPP_PRED_STENCIL)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": unknown predicate");
+  if (pred == ACPP_PRED_MASK_EQ && !std::is_integral_v<T>)
+    return set_error(ACPP_ERROR_UNSUPPORTED, std::string(name) + ": mask predicates take integer elements");
+  if (n > 0 && (!in || !out || (pred == ACPP_PRED_STENCIL && !stencil)))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": NULL pointer argument");
+  int cus = 0;
+  acpp_status st = enter_device(g, &cus);
+  if (st != ACPP_SUCCESS) return st;
+  if (n == 0) {
+    // algorithm.hpp:184-188: the count of an empty range is 0.
+    // Written by a kernel: asynchronous, and allowed under stream capture.
+    if (num_copied) {
+      hipLaunchKernelGGL(store_zero_kernel, dim3(1), dim3(1), 0, stream, num_copied);
+      st = check_launch(name);
+      if (st != ACPP_SUCCESS) return st;
+    }
+    return clear_error();
+  }
+  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 1024;
+  if (!valid_work_group_size(wg))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad work_group_size");
+  if (cfg && cfg->items_per_thread && static_cast<int>(cfg->items_per_thread) != rows<T>())
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": items_per_thread must be 8 (or 0)");
+  const uint64_t tile_elems = static_cast<uint64_t>(wg) * rows<T>() * VEC;
+  const uint64_t num_tiles = ceil_div(n, tile_elems);
+  if (num_tiles >= (1ull << 31)) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": problem too large");
+
+  void *region = nullptr;
+  size_t cap = 0;
+  region_state *rs = nullptr;
+  const size_t used = sizeof(sync_header) + num_tiles * status_stride_words * sizeof(uint64_t);
+  st = group_obtain_sync(g, used, stream, &region, &cap, &rs);
+  if (st != ACPP_SUCCESS) return st;
+  const bool capturing = stream_is_capturing(stream);
+  uint32_t epoch = 1, ticket_base = 0;
+  if (capturing) {
+    st = fill_zero(region, used, stream);
+    if (st != ACPP_SUCCESS) return st;
+    rs->mode = region_state::graph;
+  } else {
+    if (rs->mode == region_state::graph || rs->epoch >= max_epoch) {
+      st = fill_zero(region, cap, stream);
+      if (st != ACPP_SUCCESS) return st;
+      rs->epoch = 0;
+      rs->ticket_base = 0;
+    }
+    rs->mode = region_state::host_managed;
+    epoch = rs->epoch + 1;
+    ticket_base = rs->ticket_base;
+  }
+  params<T> p;
+  p.in = in;
+  p.stencil = stencil;
+  p.out = out;
+  p.num_copied = num_copied;
+  p.n = n;
+  p.pred = pred;
+  p.operand = operand;
+  p.mask = mask;
+  p.vec_io = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  p.num_tiles = static_cast<uint32_t>(num_tiles);
+  p.epoch = epoch;
+  p.ticket_base = ticket_base;
+  p.sync = static_cast<sync_header *>(region);
+  p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
+  const size_t stage_bytes = static_cast<size_t>(wg) * VEC * sizeof(T);
+  hipLaunchKernelGGL((copy_if_kernel<T, rows<T>()>), dim3(static_cast<uint32_t>(num_tiles)), dim3(wg),
+                     stage_bytes, stream, p);
+  st = check_launch(name);
+  if (st == ACPP_SUCCESS && !capturing) {
+    rs->epoch = epoch;
+    rs->ticket_base = ticket_base + static_cast<uint32_t>(num_tiles);
+  } else if (st != ACPP_SUCCESS) {
+    rs->mode = region_state::graph;
+  }
+  return st == ACPP_SUCCESS ? clear_error() : st;
+}
+
+}  // namespace copy_if_detail
+}  // namespace acpp_impl
+
+using namespace acpp_impl;
+
+extern "C" {
+
+#define ACPP_DEFINE_COPY_IF(tag, T)                                                                       \
+  acpp_status acpp_copy_if_##tag(acpp_allocation_group *scratch, acpp_stream_t stream, const T *first,   \
+                                 uint64_t n, T *d_first, int pred, T operand, T mask,                    \
+                                 uint64_t *num_copied, const acpp_launch_config *config) {               \
+    if (pred == ACPP_PRED_STENCIL)                                                                        \
+      return set_error(ACPP_ERROR_INVALID_ARGUMENT, "acpp_copy_if_" #tag ": use acpp_copy_if_stencil_" #tag); \
+    return copy_if_detail::entry<T>(scratch, static_cast<hipStream_t>(stream), first, n, nullptr, d_first,  \
+                                    pred, operand, mask, num_copied, config, "acpp_copy_if_" #tag);        \
+  }                                                                                                       \
+  acpp_status acpp_copy_if_stencil_##tag(acpp_allocation_group *scratch, acpp_stream_t stream,           \
+                                         const T *first, uint64_t n, const uint8_t *stencil, T *d_first, \
+                                         uint64_t *num_copied, const acpp_launch_config *config) {       \
+    return copy_if_detail::entry<T>(scratch, static_cast<hipStream_t>(stream), first, n, stencil, d_first,  \
+                                    ACPP_PRED_STENCIL, T(0), T(0), num_copied, config,                    \
+                                    "acpp_copy_if_stencil_" #tag);                                         \
+  }
+
+ACPP_DEFINE_COPY_IF(i32, int32_t)
+ACPP_DEFINE_COPY_IF(u32, uint32_t)
+ACPP_DEFINE_COPY_IF(i64, int64_t)
+ACPP_DEFINE_COPY_IF(u64, uint64_t)
+ACPP_DEFINE_COPY_IF(f32, float)
+ACPP_DEFINE_COPY_IF(f64, double)
+
+}  // extern "C"

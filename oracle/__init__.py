@@ -285,3 +285,21 @@ def fetch_op_indexed(rmw_op: int, values: np.ndarray, idx: np.ndarray,
                                            c_void_p(_p(idx)), c_uint64(values.size),
                                            c_void_p(_p(t)), c_uint32(t.size)))
     return t
+
+
+# copy_if predicate codes (== ACPP_PRED_*)
+PRED_NONZERO, PRED_EQ, PRED_NE, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_MASK_EQ, PRED_STENCIL = range(9)
+
+
+def copy_if(x: np.ndarray, pred: int, operand=0, mask=0, stencil: np.ndarray | None = None):
+    """(selected elements, count) -- oracle_copy_if_<t> (algorithm.hpp:178-243)."""
+    x = np.ascontiguousarray(x)
+    tag = _tag(x)
+    ct = _CT[tag]
+    out = np.zeros(max(x.size, 1), dtype=x.dtype)
+    cnt = ctypes.c_uint64(0)
+    st = None if stencil is None else np.ascontiguousarray(stencil, dtype=np.uint8)
+    fn = getattr(load(), f"oracle_copy_if_{tag}")
+    _ok(fn(c_void_p(_p(x)), c_uint64(x.size), c_void_p(_p(st) if st is not None else 0), c_int(pred),
+           ct(operand), ct(mask), c_void_p(_p(out)), ctypes.byref(cnt)))
+    return out[:cnt.value].copy(), int(cnt.value)

@@ -586,3 +586,67 @@ int oracle_fetch_op_indexed_i32(int op, const int32_t *values, const uint32_t *i
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* copy_if (include/hipSYCL/algorithms/algorithm.hpp:178-243)                */
+/* ------------------------------------------------------------------------- */
+/* The reference's structure: flags[i] = pred(x[i]) ? 1 : 0 (generator,
+   algorithm.hpp:194-205), an exclusive plus-scan of the flags (ScanT =
+   size_t, algorithm.hpp:192 and 237-242), then every selected x[i] is
+   written to d_first[scan[i]] and the last element reports
+   scan[n-1] + flag[n-1] as the number copied (algorithm.hpp:207-234).
+   Predicate codes: 0 x != 0, 1 ==, 2 !=, 3 <, 4 <=, 5 >, 6 >=, 7 (x & mask)
+   == operand, 8 stencil[i] != 0.  Fault 6 drops the last selected element
+   from the count. */
+#define DEF_COPY_IF(t, T, IS_INT)                                                             \
+  static int pred_##t(int pred, T x, T operand, T mask, const uint8_t *stencil, uint64_t i) { \
+    (void)mask;                                                                             \
+    switch (pred) {                                                                         \
+      case 0: return x != (T)0;                                                             \
+      case 1: return x == operand;                                                          \
+      case 2: return x != operand;                                                          \
+      case 3: return x < operand;                                                           \
+      case 4: return x <= operand;                                                          \
+      case 5: return x > operand;                                                           \
+      case 6: return x >= operand;                                                          \
+      case 7: return IS_INT ? (COPY_IF_MASK(T, x, mask) == operand) : 0;                     \
+      default: return stencil[i] != 0;                                                      \
+    }                                                                                       \
+  }                                                                                         \
+  int oracle_copy_if_##t(const T *x, uint64_t n, const uint8_t *stencil, int pred, T operand, \
+                         T mask, T *out, uint64_t *count) {                                 \
+    if (n == 0) {                                                                           \
+      *count = 0;                                                                           \
+      return 0;                                                                             \
+    }                                                                                       \
+    uint64_t *scan = (uint64_t *)malloc(n * sizeof(uint64_t));                              \
+    if (!scan) return -1;                                                                   \
+    uint64_t acc = 0;                                                                       \
+    for (uint64_t i = 0; i < n; ++i) {                                                      \
+      scan[i] = acc;                                                                        \
+      acc += pred_##t(pred, x[i], operand, mask, stencil, i) ? 1u : 0u;                      \
+    }                                                                                       \
+    int last = 0;                                                                           \
+    for (uint64_t i = 0; i < n; ++i) {                                                      \
+      last = pred_##t(pred, x[i], operand, mask, stencil, i);                               \
+      if (last) out[scan[i]] = x[i];                                                        \
+    }                                                                                       \
+    *count = scan[n - 1] + (uint64_t)last;                                                  \
+    if (g_fault == 6 && *count > 0) *count -= 1;                                            \
+    free(scan);                                                                             \
+    return 0;                                                                               \
+  }
+
+#define COPY_IF_MASK(T, x, mask) COPY_IF_MASK_##T(x, mask)
+#define COPY_IF_MASK_int32_t(x, mask) ((int32_t)((x) & (mask)))
+#define COPY_IF_MASK_uint32_t(x, mask) ((uint32_t)((x) & (mask)))
+#define COPY_IF_MASK_int64_t(x, mask) ((int64_t)((x) & (mask)))
+#define COPY_IF_MASK_uint64_t(x, mask) ((uint64_t)((x) & (mask)))
+#define COPY_IF_MASK_float(x, mask) ((float)0)
+#define COPY_IF_MASK_double(x, mask) ((double)0)
+DEF_COPY_IF(i32, int32_t, 1)
+DEF_COPY_IF(u32, uint32_t, 1)
+DEF_COPY_IF(i64, int64_t, 1)
+DEF_COPY_IF(u64, uint64_t, 1)
+DEF_COPY_IF(f32, float, 0)
+DEF_COPY_IF(f64, double, 0)

@@ -83,6 +83,16 @@ int oracle_fetch_max_strided_i32(const int32_t *values, uint64_t n, int32_t *slo
 int oracle_fetch_op_indexed_i32(int op, const int32_t *values, const uint32_t *idx, uint64_t n,
                                 int32_t *target, uint32_t num_targets);
 
+/* copy_if: out[0..count) = the x[i] (in order) for which the predicate
+   holds (codes: 0 x != 0, 1 ==, 2 !=, 3 <, 4 <=, 5 >, 6 >=, 7 (x & mask) ==
+   operand, 8 stencil[i] != 0). */
+int oracle_copy_if_i32(const int32_t *x, uint64_t n, const uint8_t *stencil, int pred, int32_t operand, int32_t mask, int32_t *out, uint64_t *count);
+int oracle_copy_if_u32(const uint32_t *x, uint64_t n, const uint8_t *stencil, int pred, uint32_t operand, uint32_t mask, uint32_t *out, uint64_t *count);
+int oracle_copy_if_i64(const int64_t *x, uint64_t n, const uint8_t *stencil, int pred, int64_t operand, int64_t mask, int64_t *out, uint64_t *count);
+int oracle_copy_if_u64(const uint64_t *x, uint64_t n, const uint8_t *stencil, int pred, uint64_t operand, uint64_t mask, uint64_t *out, uint64_t *count);
+int oracle_copy_if_f32(const float *x, uint64_t n, const uint8_t *stencil, int pred, float operand, float mask, float *out, uint64_t *count);
+int oracle_copy_if_f64(const double *x, uint64_t n, const uint8_t *stencil, int pred, double operand, double mask, double *out, uint64_t *count);
+
 /* Number of OpenMP threads a parallel region would use by default. */
 int oracle_max_threads(void);
 
