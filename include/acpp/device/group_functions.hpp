@@ -407,6 +407,149 @@ __device__ OutPtr joint_exclusive_scan(group g, InPtr first, InPtr last, OutPtr 
   return joint_exclusive_scan(g, first, last, result, known_identity<F, T>::get(), op);
 }
 
+// ---------------------------------------------------------------------------
+// Broadcast, barrier, any/all/none, shifts, permutes and selects
+// (include/hipSYCL/sycl/libkernel/group_functions.hpp:57-140, 240-276;
+// the HIP implementation is generic/hiplike/group_functions.hpp:75-212 and
+// 546-640: shuffles for sub-groups, an LDS array of group size for groups).
+//
+// Sub-group forms move 32-bit pieces with ds_bpermute (any source lane) or
+// readlane (a uniform source lane) and take the ballot for any/all/none.
+// Work-group forms use __syncthreads_or / _and for the predicates and one
+// LDS slot per work-item for the data movement.
+//
+// A source outside the group gives the caller's own value for a sub-group
+// (as the HIP shuffles do) and work-item 0's value for a work-group (the
+// reference reads slot 0; it also reads slot `size` when the target is
+// exactly the group size, which is outside the written range -- here that
+// case reads slot 0 as well).
+// ---------------------------------------------------------------------------
+namespace detail {
+
+template <class T>
+__device__ __forceinline__ T bpermute(const T &x, uint32_t src_lane) {
+  constexpr int words = (sizeof(T) + 3) / 4;
+  uint32_t w[words] = {};
+  __builtin_memcpy(w, &x, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < words; ++i)
+    w[i] = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(static_cast<int>(src_lane << 2),
+                                                              static_cast<int>(w[i])));
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+// Read the value of `src` in the sub-group, or keep x when src is outside.
+template <class T>
+__device__ __forceinline__ T sub_group_fetch(sub_group g, const T &x, uint64_t src) {
+  const bool ok = src < g.get_local_linear_range();
+  const T y = bpermute(x, static_cast<uint32_t>(ok ? src : g.get_local_linear_id()));
+  return ok ? y : x;
+}
+
+// Work-group data movement through one LDS slot per work-item.
+template <class T>
+__device__ __forceinline__ T group_fetch(group g, const T &x, uint64_t src) {
+  __shared__ raw_slot<T> slots[1024];
+  const uint32_t lid = g.get_local_linear_id();
+  __syncthreads();  // slots may still be read by a previous collective
+  store_slot(slots[lid], x);
+  __syncthreads();
+  return load_slot(slots[src < g.get_local_linear_range() ? src : 0]);
+}
+
+}  // namespace detail
+
+template <class T>
+__device__ T group_broadcast(sub_group, T x, uint32_t local_linear_id = 0) {
+  return detail::read_lane_any(x, static_cast<int>(local_linear_id));
+}
+template <class T>
+__device__ T group_broadcast(group g, T x, uint32_t local_linear_id = 0) {
+  __shared__ detail::raw_slot<T> slot;
+  __syncthreads();
+  if (g.get_local_linear_id() == local_linear_id) detail::store_slot(slot, x);
+  __syncthreads();
+  return detail::load_slot(slot);
+}
+
+__device__ inline void group_barrier(group) { __syncthreads(); }
+__device__ inline void group_barrier(sub_group) { __builtin_amdgcn_wave_barrier(); }
+
+__device__ inline bool any_of_group(sub_group, bool pred) { return wave::ballot(pred) != 0; }
+__device__ inline bool all_of_group(sub_group, bool pred) {
+  return wave::ballot(!pred) == 0;  // inactive lanes do not vote
+}
+__device__ inline bool none_of_group(sub_group g, bool pred) { return !any_of_group(g, pred); }
+__device__ inline bool any_of_group(group, bool pred) { return __syncthreads_or(pred) != 0; }
+__device__ inline bool all_of_group(group, bool pred) { return __syncthreads_and(pred) != 0; }
+__device__ inline bool none_of_group(group g, bool pred) { return !any_of_group(g, pred); }
+
+template <class G, class T, class P>
+__device__ bool any_of_group(G g, T x, P pred) { return any_of_group(g, static_cast<bool>(pred(x))); }
+template <class G, class T, class P>
+__device__ bool all_of_group(G g, T x, P pred) { return all_of_group(g, static_cast<bool>(pred(x))); }
+template <class G, class T, class P>
+__device__ bool none_of_group(G g, T x, P pred) { return none_of_group(g, static_cast<bool>(pred(x))); }
+
+// joint_*_of: the group checks [first, last) together, strided by the
+// group size (HGF __acpp_joint_any_of and friends).
+template <class G, class Ptr, class P>
+__device__ bool joint_any_of(G g, Ptr first, Ptr last, P pred) {
+  bool found = false;
+  for (Ptr p = first + g.get_local_linear_id(); p < last; p += g.get_local_linear_range())
+    found = found || static_cast<bool>(pred(*p));
+  return any_of_group(g, found);
+}
+template <class G, class Ptr, class P>
+__device__ bool joint_all_of(G g, Ptr first, Ptr last, P pred) {
+  bool all = true;
+  for (Ptr p = first + g.get_local_linear_id(); p < last; p += g.get_local_linear_range())
+    all = all && static_cast<bool>(pred(*p));
+  return all_of_group(g, all);
+}
+template <class G, class Ptr, class P>
+__device__ bool joint_none_of(G g, Ptr first, Ptr last, P pred) {
+  return !joint_any_of(g, first, last, pred);
+}
+
+template <class T>
+__device__ T shift_group_left(sub_group g, T x, uint32_t delta = 1) {
+  return detail::sub_group_fetch(g, x, static_cast<uint64_t>(g.get_local_linear_id()) + delta);
+}
+template <class T>
+__device__ T shift_group_right(sub_group g, T x, uint32_t delta = 1) {
+  const uint32_t l = g.get_local_linear_id();
+  return detail::sub_group_fetch(g, x, l >= delta ? static_cast<uint64_t>(l - delta) : ~0ull);
+}
+template <class T>
+__device__ T permute_group_by_xor(sub_group g, T x, uint32_t mask) {
+  return detail::sub_group_fetch(g, x, static_cast<uint64_t>(g.get_local_linear_id() ^ mask));
+}
+template <class T>
+__device__ T select_from_group(sub_group g, T x, uint32_t remote_local_id) {
+  return detail::sub_group_fetch(g, x, remote_local_id);
+}
+
+template <class T>
+__device__ T shift_group_left(group g, T x, uint32_t delta = 1) {
+  return detail::group_fetch(g, x, static_cast<uint64_t>(g.get_local_linear_id()) + delta);
+}
+template <class T>
+__device__ T shift_group_right(group g, T x, uint32_t delta = 1) {
+  const uint32_t l = g.get_local_linear_id();
+  return detail::group_fetch(g, x, l >= delta ? static_cast<uint64_t>(l - delta) : ~0ull);
+}
+template <class T>
+__device__ T permute_group_by_xor(group g, T x, uint32_t mask) {
+  return detail::group_fetch(g, x, static_cast<uint64_t>(g.get_local_linear_id() ^ mask));
+}
+template <class T>
+__device__ T select_from_group(group g, T x, uint32_t remote_local_id) {
+  return detail::group_fetch(g, x, remote_local_id);
+}
+
 }  // namespace acpp
 
 // ---------------------------------------------------------------------------

@@ -267,6 +267,74 @@ int launch_joint(int kind, const void *in, void *out, uint64_t n, uint64_t seg_l
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
+// Broadcast / predicates / data movement.  level 0: sub-group (wave), 1:
+// work-group.  which: 0 broadcast(param), 1 shift_left(param),
+// 2 shift_right(param), 3 permute_by_xor(param), 4 select_from_group(src[i]),
+// 5 any_of(x > param), 6 all_of(x > param), 7 none_of(x > param),
+// 8 joint_any_of / 9 joint_all_of / 10 joint_none_of over the work-group's
+// slice of `in` (x > param).
+template <class T>
+__global__ void movement_kernel(int level, int which, uint32_t param, const T *in, const uint32_t *src,
+                                T *out) {
+  const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const T x = in[gid];
+  const T thr = static_cast<T>(static_cast<int32_t>(param));  // thresholds may be negative
+  auto gt = [thr](T v) { return v > thr; };
+  T r{};
+  if (level == 0) {
+    acpp::sub_group g = acpp::this_sub_group();
+    switch (which) {
+      case 0: r = acpp::group_broadcast(g, x, param); break;
+      case 1: r = acpp::shift_group_left(g, x, param); break;
+      case 2: r = acpp::shift_group_right(g, x, param); break;
+      case 3: r = acpp::permute_group_by_xor(g, x, param); break;
+      case 4: r = acpp::select_from_group(g, x, src[gid]); break;
+      case 5: r = acpp::any_of_group(g, x, gt) ? T(1) : T(0); break;
+      case 6: r = acpp::all_of_group(g, x, gt) ? T(1) : T(0); break;
+      case 7: r = acpp::none_of_group(g, x, gt) ? T(1) : T(0); break;
+      default: {
+        const T *first = in + static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u);
+        const T *last = first + g.get_local_linear_range();
+        bool b = which == 8 ? acpp::joint_any_of(g, first, last, gt)
+                            : which == 9 ? acpp::joint_all_of(g, first, last, gt)
+                                         : acpp::joint_none_of(g, first, last, gt);
+        r = b ? T(1) : T(0);
+      }
+    }
+  } else {
+    acpp::group g = acpp::this_group();
+    switch (which) {
+      case 0: r = acpp::group_broadcast(g, x, param); break;
+      case 1: r = acpp::shift_group_left(g, x, param); break;
+      case 2: r = acpp::shift_group_right(g, x, param); break;
+      case 3: r = acpp::permute_group_by_xor(g, x, param); break;
+      case 4: r = acpp::select_from_group(g, x, src[gid]); break;
+      case 5: r = acpp::any_of_group(g, x, gt) ? T(1) : T(0); break;
+      case 6: r = acpp::all_of_group(g, x, gt) ? T(1) : T(0); break;
+      case 7: r = acpp::none_of_group(g, x, gt) ? T(1) : T(0); break;
+      default: {
+        const T *first = in + static_cast<uint64_t>(blockIdx.x) * blockDim.x;
+        const T *last = first + blockDim.x;
+        bool b = which == 8 ? acpp::joint_any_of(g, first, last, gt)
+                            : which == 9 ? acpp::joint_all_of(g, first, last, gt)
+                                         : acpp::joint_none_of(g, first, last, gt);
+        r = b ? T(1) : T(0);
+      }
+    }
+  }
+  out[gid] = r;
+}
+
+template <class T>
+int launch_movement(int level, int which, uint32_t param, const void *in, const void *src, void *out,
+                    uint64_t n, uint32_t local_size, hipStream_t s) {
+  if (local_size == 0 || local_size > 1024 || n % local_size != 0) return 1;
+  hipLaunchKernelGGL(movement_kernel<T>, dim3(static_cast<uint32_t>(n / local_size)), dim3(local_size), 0, s,
+                     level, which, param, static_cast<const T *>(in), static_cast<const uint32_t *>(src),
+                     static_cast<T *>(out));
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
 }  // namespace
 
 extern "C" {
@@ -293,6 +361,19 @@ int tk_joint(int tag, int kind, const void *in, void *out, uint64_t n, uint64_t 
     case 1: return launch_joint<int32_t>(kind, in, out, n, seg_len, local_size, op, init, s);
     case 2: return launch_joint<int64_t>(kind, in, out, n, seg_len, local_size, op, init, s);
     case 4: return launch_joint<float>(kind, in, out, n, seg_len, local_size, op, init, s);
+    default: return 1;
+  }
+}
+
+// tag: 1 i32, 2 i64, 4 f32, 5 f64; n a multiple of local_size
+int tk_group_movement(int tag, int level, int which, uint32_t param, const void *in, const void *src,
+                      void *out, uint64_t n, uint32_t local_size, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (tag) {
+    case 1: return launch_movement<int32_t>(level, which, param, in, src, out, n, local_size, s);
+    case 2: return launch_movement<int64_t>(level, which, param, in, src, out, n, local_size, s);
+    case 4: return launch_movement<float>(level, which, param, in, src, out, n, local_size, s);
+    case 5: return launch_movement<double>(level, which, param, in, src, out, n, local_size, s);
     default: return 1;
   }
 }
