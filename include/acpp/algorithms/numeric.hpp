@@ -112,14 +112,22 @@ ACPP_FACADE_DOT(float, f32, float)
 ACPP_FACADE_DOT(double, f64, double)
 #undef ACPP_FACADE_DOT
 
-// Contiguous iterators: pointers and std::vector iterators.
+// Contiguous iterators: pointers, and the pointer-wrapping iterators of
+// contiguous standard containers (std::vector with any allocator).
+template <class It>
+struct wraps_pointer : std::false_type {};
+#if defined(__GLIBCXX__)
+template <class P, class C>
+struct wraps_pointer<__gnu_cxx::__normal_iterator<P, C>> : std::is_pointer<P> {};
+#endif
+
 template <class It>
 struct contiguous {
   using value = std::remove_cv_t<typename std::iterator_traits<It>::value_type>;
   static constexpr bool pointer = std::is_pointer<It>::value;
   static constexpr bool vector_it =
       std::is_same<It, typename std::vector<value>::iterator>::value ||
-      std::is_same<It, typename std::vector<value>::const_iterator>::value;
+      std::is_same<It, typename std::vector<value>::const_iterator>::value || wraps_pointer<It>::value;
   static constexpr bool value_ok = pointer || vector_it;
 };
 

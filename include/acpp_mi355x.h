@@ -314,6 +314,22 @@ acpp_status acpp_segmented_plan_launch(acpp_segmented_plan *plan, acpp_stream_t 
 acpp_status acpp_segmented_plan_destroy(acpp_segmented_plan *plan);
 
 /* ------------------------------------------------------------------------ */
+/* Several reductions in one kernel: parallel_for with up to four           */
+/* sycl::reduction objects (include/hipSYCL/sycl/handler.hpp:799-975;        */
+/* tests/sycl/reduction.cpp:116-208).  outs[k] = fold of inputs[k] with      */
+/* ops[k]; with initialize_to_identity the fold starts from the identity,  */
+/* otherwise from *outs[k] (reduction_descriptor.hpp:36-53).  `inputs`,     */
+/* `outs` and `ops` are host arrays of `count` (1..4) entries; the pointers */
+/* in them are device pointers; inputs may alias.  One launch.              */
+/* ------------------------------------------------------------------------ */
+acpp_status acpp_reduction_multi_i32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int32_t *const *inputs, uint64_t n, int32_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_u32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint32_t *const *inputs, uint64_t n, uint32_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_i64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int64_t *const *inputs, uint64_t n, int64_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_u64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint64_t *const *inputs, uint64_t n, uint64_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_f32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const float *const *inputs, uint64_t n, float *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_f64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const double *const *inputs, uint64_t n, double *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+
+/* ------------------------------------------------------------------------ */
 /* copy_if: stream compaction                                               */
 /* Replaces algorithms::copy_if (include/hipSYCL/algorithms/algorithm.hpp:  */
 /* 178-243): the elements x[i] for which the predicate holds are written,   */

@@ -13,7 +13,7 @@ from .algorithms import (ALLOCATION_DEVICE, ALLOCATION_HOST, ALLOCATION_SHARED, 
                          exclusive_scan, inclusive_scan, reduce, reduce_compensated, reduction,
                          transform_reduce)
 from .algorithms import (PRED_EQ, PRED_GE, PRED_GT, PRED_LE, PRED_LT, PRED_MASK_EQ,  # noqa: F401
-                         PRED_NE, PRED_NONZERO, copy_if, copy_if_stencil)
+                         PRED_NE, PRED_NONZERO, copy_if, copy_if_stencil, reduction_multi)
 from .cost_model import CostModel  # noqa: F401
 
 __version__ = "0.1.0"

@@ -244,6 +244,29 @@ def reduction(q: Queue, scratch: AllocationGroup, x, out, op=PLUS,
              1 if initialize_to_identity else 0, _cfg(config)))
 
 
+def reduction_multi(q: Queue, scratch: AllocationGroup, inputs, outs, ops,
+                    initialize_to_identity: bool = False, config=None) -> None:
+    """Up to four sycl::reduction objects in one parallel_for
+    (handler.hpp:799-975): outs[k][0] = fold of inputs[k] with ops[k]
+    (starting from the identity, or from outs[k][0]).  Inputs share one
+    length and element type; they may be the same tensor."""
+    k = len(inputs)
+    if not (1 <= k <= 4) or len(outs) != k or len(ops) != k:
+        raise ValueError("one to four (input, output, operator) triples")
+    n = inputs[0].numel()
+    tag = _tag(inputs[0].dtype)
+    for x, o in zip(inputs, outs):
+        _check_contig(x, o)
+        if x.numel() != n or _tag(x.dtype) != tag or _tag(o.dtype) != tag:
+            raise ValueError("inputs must share length and element type with the outputs")
+    arr_in = (ctypes.c_void_p * k)(*[_ptr(x) for x in inputs])
+    arr_out = (ctypes.c_void_p * k)(*[_ptr(o) for o in outs])
+    arr_ops = (ctypes.c_int32 * k)(*[_op(o) for o in ops])
+    fn = getattr(_abi.load(), f"acpp_reduction_multi_{tag}")
+    check(fn(scratch.handle, q.handle, k, arr_in, n, arr_out, arr_ops, int(bool(initialize_to_identity)),
+             _cfg(config)))
+
+
 def inclusive_scan(q: Queue, scratch: AllocationGroup, x, d, op=PLUS, init=None,
                    config=None) -> None:
     """algorithms::inclusive_scan (numeric.hpp:253-279)."""

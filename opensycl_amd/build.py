@@ -130,23 +130,30 @@ def build_bench_helpers() -> str:
     return out
 
 
-def build_cpp_tests() -> str:
-    """Build tests/lib/test_facade: the C++ facade test (host code only,
-    compiled with g++ against the library and the HIP runtime)."""
-    src = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+def build_cpp_tests() -> list[str]:
+    """Build tests/lib/<name> for every tests/cpp/<name>.cpp: C++ tests of the
+    facade and the stdpar entry points (host code only, compiled with g++
+    against the library and the HIP runtime)."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
     out_dir = os.path.join(ROOT, "tests", "lib")
     os.makedirs(out_dir, exist_ok=True)
-    out = os.path.join(out_dir, "test_facade")
     rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", f"-I{rocm}/include",
-           "-D__HIP_PLATFORM_AMD__", src, "-o", out + ".tmp", f"-L{LIB_DIR}", "-lacpp_mi355x",
-           f"-L{rocm}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../opensycl_amd/lib",
-           f"-Wl,-rpath,{rocm}/lib"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("C++ facade test build failed:\n" + r.stderr)
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for name in sorted(os.listdir(src_dir)):
+        if not name.endswith(".cpp"):
+            continue
+        src = os.path.join(src_dir, name)
+        out = os.path.join(out_dir, name[:-4])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", f"-I{rocm}/include",
+               "-D__HIP_PLATFORM_AMD__", src, "-o", out + ".tmp", f"-L{LIB_DIR}", "-lacpp_mi355x",
+               f"-L{rocm}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../opensycl_amd/lib",
+               f"-Wl,-rpath,{rocm}/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ test {name} build failed:\n" + r.stderr)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_oracle() -> str:

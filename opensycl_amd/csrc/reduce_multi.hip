@@ -1,0 +1,234 @@
+// Several sycl::reduction objects in one kernel (handler::parallel_for with
+// more than one reduction, include/hipSYCL/sycl/handler.hpp:799-975; test
+// tests/sycl/reduction.cpp:116-208 runs a plus and a multiplies reduction in
+// one parallel_for).
+//
+// One launch reduces up to four inputs of the same length, each with its
+// own operator and output: every work-group streams the inputs one after
+// the other (the operator switch sits outside each streaming loop, so the
+// loops are the same tight 16-byte-load loops as the single reduction),
+// reduces each with DPP and one LDS exchange, and publishes one partial per
+// input; the last work-group to finish (fence-free ticket, as in
+// reduce_common.hpp) folds every input's partials in a fixed order and
+// writes the results.  Inputs may alias (several operators over one array).
+#include "launch_util.hpp"
+#include "reduce_common.hpp"
+
+namespace acpp_impl {
+namespace multi_detail {
+
+using reduce_detail::block_reduce;
+using reduce_detail::max_waves;
+using reduce_detail::load_partial;
+using reduce_detail::plain_monoid;
+using reduce_detail::store_partial;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int max_inputs = 4;
+constexpr int unroll = 4;
+
+template <class T>
+struct params {
+  using W = typename elem_traits<T>::work;
+  const T *in[max_inputs];
+  T *out[max_inputs];
+  int op[max_inputs];
+  int count;
+  uint64_t n;
+  int identity_init;  // 1: out = fold; 0: out = op(out, fold)
+  W *partials;        // [count][gridDim.x]
+  sync_header *sync;
+};
+
+// Calls f(Op{}) with the operator functor for `code`.
+template <class W, class F>
+__device__ __forceinline__ void dispatch(int code, F &&f) {
+  switch (code) {
+    case ops::op_plus: f(ops::plus<W, W>{}); return;
+    case ops::op_multiply: f(ops::multiply<W, W>{}); return;
+    case ops::op_min: f(ops::minimum<W, W>{}); return;
+    case ops::op_max: f(ops::maximum<W, W>{}); return;
+    default: break;
+  }
+  if constexpr (std::is_integral_v<W>) {
+    switch (code) {
+      case ops::op_bit_and: f(ops::bit_and<W, W>{}); return;
+      case ops::op_bit_or: f(ops::bit_or<W, W>{}); return;
+      case ops::op_bit_xor: f(ops::bit_xor<W, W>{}); return;
+      case ops::op_logical_and: f(ops::logical_and<W, W>{}); return;
+      default: f(ops::logical_or<W, W>{}); return;
+    }
+  }
+}
+
+template <class T, class Op>
+__device__ typename elem_traits<T>::work stream_fold(const T *in, uint64_t n) {
+  using W = typename elem_traits<T>::work;
+  constexpr int VEC = 16 / sizeof(T);
+  const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  W acc = Op::identity();
+  uint64_t scalar_from = 0;
+  if ((reinterpret_cast<uintptr_t>(in) & 15) == 0) {
+    const uint64_t nvec = n / VEC;
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(in);
+    for (uint64_t i = gid; i < nvec; i += stride * unroll) {
+      u32x4 q[unroll];
+#pragma unroll
+      for (int u = 0; u < unroll; ++u) {
+        const uint64_t j = i + u * stride;
+        if (j < nvec) q[u] = src[j];
+      }
+#pragma unroll
+      for (int u = 0; u < unroll; ++u) {
+        if (i + u * stride < nvec) {
+          T t[VEC];
+          __builtin_memcpy(t, &q[u], 16);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) acc = Op::apply(acc, elem_traits<T>::to_work(t[k]));
+        }
+      }
+    }
+    scalar_from = nvec * VEC;
+  }
+  for (uint64_t i = scalar_from + gid; i < n; i += stride) acc = Op::apply(acc, elem_traits<T>::to_work(in[i]));
+  return acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void multi_kernel(params<T> p) {
+  using W = typename elem_traits<T>::work;
+  __shared__ W s_waves[max_waves];
+  __shared__ bool s_last;
+  for (int k = 0; k < p.count; ++k) {
+    __syncthreads();  // s_waves of the previous input has been read
+    dispatch<W>(p.op[k], [&](auto opv) {
+      using Op = decltype(opv);
+      W v = stream_fold<T, Op>(p.in[k], p.n);
+      v = block_reduce<plain_monoid<W, Op>>(v, s_waves);
+      if (threadIdx.x == 0) store_partial(p.partials + static_cast<uint64_t>(k) * gridDim.x + blockIdx.x, v);
+    });
+  }
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_s_waitcnt(0);  // partials stored before the ticket
+    const uint32_t prev =
+        __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  for (int k = 0; k < p.count; ++k) {
+    __syncthreads();  // s_waves of the previous input has been read
+    dispatch<W>(p.op[k], [&](auto opv) {
+      using Op = decltype(opv);
+      W r = Op::identity();
+      for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+        r = Op::apply(r, load_partial(p.partials + static_cast<uint64_t>(k) * gridDim.x + i));
+      r = block_reduce<plain_monoid<W, Op>>(r, s_waves);
+      if (threadIdx.x == 0) {
+        if (!p.identity_init) r = Op::apply(elem_traits<T>::to_work(*p.out[k]), r);
+        *p.out[k] = static_cast<T>(r);
+      }
+    });
+  }
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&p.sync->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// n == 0 with initialize_to_identity: every output becomes its identity.
+template <class T>
+__global__ void identity_kernel(params<T> p) {
+  using W = typename elem_traits<T>::work;
+  const int k = threadIdx.x;
+  if (k < p.count)
+    dispatch<W>(p.op[k], [&](auto opv) { *p.out[k] = static_cast<T>(decltype(opv)::identity()); });
+}
+
+template <class T>
+acpp_status entry(acpp_allocation_group *g, hipStream_t stream, uint32_t count, const T *const *inputs,
+                  uint64_t n, T *const *outs, const acpp_algorithm_op *op_codes, int identity_init,
+                  const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (count < 1 || count > static_cast<uint32_t>(max_inputs))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": count must be 1..4");
+  if (!inputs || !outs || !op_codes)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": NULL array argument");
+  params<T> p{};
+  p.count = static_cast<int>(count);
+  p.n = n;
+  p.identity_init = identity_init ? 1 : 0;
+  for (uint32_t k = 0; k < count; ++k) {
+    const int op = op_codes[k];
+    if (op < ops::op_plus || op > ops::op_logical_or)
+      return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": unknown operator");
+    if (std::is_floating_point_v<W> && op > ops::op_max)
+      return set_error(ACPP_ERROR_UNSUPPORTED, std::string(name) + ": floating-point reductions take plus, "
+                                                                   "multiply, min or max");
+    if (!outs[k] || (n > 0 && !inputs[k]))
+      return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": NULL input or output");
+    p.in[k] = inputs[k];
+    p.out[k] = outs[k];
+    p.op[k] = op;
+  }
+  int cus = 0;
+  acpp_status st = enter_device(g, &cus);
+  if (st != ACPP_SUCCESS) return st;
+  if (n == 0) {
+    // reduction_descriptor.hpp:36-53: with initialize_to_identity the output
+    // is the identity; without it the output keeps its value.
+    if (p.identity_init) {
+      hipLaunchKernelGGL(identity_kernel<T>, dim3(1), dim3(64), 0, stream, p);
+      st = check_launch(name);
+      if (st != ACPP_SUCCESS) return st;
+    }
+    return clear_error();
+  }
+  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : 512;
+  if (!valid_work_group_size(wg))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad work_group_size");
+  int resident = 1;
+  st = occupancy(multi_kernel<T>, static_cast<int>(wg), &resident);
+  if (st != ACPP_SUCCESS) return st;
+  const uint32_t gpc = (cfg && cfg->groups_per_cu) ? cfg->groups_per_cu : 1;
+  const uint64_t cap = static_cast<uint64_t>(std::min<uint32_t>(gpc, static_cast<uint32_t>(resident))) * cus;
+  constexpr int VEC = 16 / sizeof(T);
+  const uint64_t needed = ceil_div(ceil_div(n, static_cast<uint64_t>(VEC)), static_cast<uint64_t>(wg) * unroll);
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(cap, needed)));
+  void *partials = nullptr;
+  st = group_obtain(g, static_cast<size_t>(grid) * count * sizeof(W), &partials);
+  if (st != ACPP_SUCCESS) return st;
+  void *sync = nullptr;
+  size_t cap_bytes = 0;
+  st = group_obtain_sync(g, sizeof(sync_header), stream, &sync, &cap_bytes);
+  if (st != ACPP_SUCCESS) return st;
+  p.partials = static_cast<W *>(partials);
+  p.sync = static_cast<sync_header *>(sync);
+  hipLaunchKernelGGL(multi_kernel<T>, dim3(grid), dim3(wg), 0, stream, p);
+  st = check_launch(name);
+  return st == ACPP_SUCCESS ? clear_error() : st;
+}
+
+}  // namespace multi_detail
+}  // namespace acpp_impl
+
+using namespace acpp_impl;
+
+extern "C" {
+
+#define ACPP_DEFINE_MULTI(tag, T)                                                                       \
+  acpp_status acpp_reduction_multi_##tag(acpp_allocation_group *scratch, acpp_stream_t stream,         \
+                                         uint32_t count, const T *const *inputs, uint64_t n,           \
+                                         T *const *outs, const acpp_algorithm_op *ops,                 \
+                                         int initialize_to_identity, const acpp_launch_config *config) { \
+    return multi_detail::entry<T>(scratch, static_cast<hipStream_t>(stream), count, inputs, n, outs, ops, \
+                                  initialize_to_identity, config, "acpp_reduction_multi_" #tag);       \
+  }
+ACPP_DEFINE_MULTI(i32, int32_t)
+ACPP_DEFINE_MULTI(u32, uint32_t)
+ACPP_DEFINE_MULTI(i64, int64_t)
+ACPP_DEFINE_MULTI(u64, uint64_t)
+ACPP_DEFINE_MULTI(f32, float)
+ACPP_DEFINE_MULTI(f64, double)
+
+}  // extern "C"

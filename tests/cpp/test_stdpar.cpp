@@ -1,0 +1,116 @@
+// stdpar entry points (include/acpp/stdpar/numeric.hpp) on std::vector data
+// in device-accessible (pinned host) memory, checked against the sequential
+// C++ standard library, with the cases of the reference's pstl tests:
+//   tests/pstl/reduce.cpp:24-66 (sizes 0, 1, 127, 1000, 1000*1000; init 10 / 0 / 0ll),
+//   tests/pstl/transform_reduce.cpp, inclusive_scan.cpp, exclusive_scan.cpp.
+// Exit status 0 = all passed.
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "acpp/stdpar/numeric.hpp"
+
+namespace sp = acpp::stdpar;
+
+template <class T>
+struct pinned_allocator {
+  using value_type = T;
+  pinned_allocator() = default;
+  template <class U>
+  pinned_allocator(const pinned_allocator<U> &) {}
+  T *allocate(std::size_t n) {
+    void *p = nullptr;
+    acpp::detail::check_hip(hipHostMalloc(&p, (n ? n : 1) * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    return static_cast<T *>(p);
+  }
+  void deallocate(T *p, std::size_t) { (void)hipHostFree(p); }
+  template <class U>
+  bool operator==(const pinned_allocator<U> &) const { return true; }
+  template <class U>
+  bool operator!=(const pinned_allocator<U> &) const { return false; }
+};
+template <class T>
+using pvec = std::vector<T, pinned_allocator<T>>;
+
+static int failures = 0;
+#define CHECK(cond, ...)                 \
+  do {                                   \
+    if (!(cond)) {                       \
+      std::printf("FAIL: " __VA_ARGS__); \
+      std::printf("\n");                \
+      ++failures;                        \
+    }                                    \
+  } while (0)
+
+template <class T>
+void test_basic_reduction(T init, std::size_t size) {
+  pvec<T> data(size);
+  for (std::size_t i = 0; i < data.size(); ++i) data[i] = static_cast<T>(i);
+  T ref = std::reduce(data.begin(), data.end(), init, std::plus<>{});
+  T res = sp::reduce(sp::par_unseq, data.begin(), data.end(), init, std::plus<>{});
+  CHECK(res == ref, "reduce size %zu", size);
+  T res2 = sp::reduce(sp::par_unseq, data.begin(), data.end(), init);
+  CHECK(res2 == res, "reduce(init) size %zu", size);
+  T ref3 = std::reduce(data.begin(), data.end());
+  T res3 = sp::reduce(sp::par_unseq, data.begin(), data.end());
+  CHECK(res3 == ref3, "reduce() size %zu", size);
+}
+
+template <class T>
+void test_transform_reduce(std::size_t size) {
+  pvec<T> a(size), b(size);
+  for (std::size_t i = 0; i < size; ++i) {
+    a[i] = static_cast<T>(i % 13);
+    b[i] = static_cast<T>(i % 7);
+  }
+  T ref = std::transform_reduce(a.begin(), a.end(), b.begin(), T(3));
+  T res = sp::transform_reduce(sp::par_unseq, a.begin(), a.end(), b.begin(), T(3));
+  CHECK(res == ref, "transform_reduce size %zu", size);
+  T res2 = sp::transform_reduce(sp::par_unseq, a.begin(), a.end(), b.begin(), T(3), std::plus<T>{},
+                                std::multiplies<T>{});
+  CHECK(res2 == ref, "transform_reduce(ops) size %zu", size);
+}
+
+template <class T>
+void test_scans(std::size_t size) {
+  pvec<T> x(size), y(size), z(size);
+  for (std::size_t i = 0; i < size; ++i) x[i] = static_cast<T>((i * 7) % 19) - 9;
+  std::vector<T> ry(size), rz(size);
+  std::inclusive_scan(x.begin(), x.end(), ry.begin());
+  auto e1 = sp::inclusive_scan(sp::par_unseq, x.begin(), x.end(), y.begin());
+  CHECK(e1 == y.end(), "inclusive_scan end size %zu", size);
+  CHECK(std::equal(y.begin(), y.end(), ry.begin()), "inclusive_scan size %zu", size);
+  std::inclusive_scan(x.begin(), x.end(), ry.begin(), std::plus<>{}, T(5));
+  sp::inclusive_scan(sp::par_unseq, x.begin(), x.end(), y.begin(), std::plus<>{}, T(5));
+  CHECK(std::equal(y.begin(), y.end(), ry.begin()), "inclusive_scan(init) size %zu", size);
+  std::exclusive_scan(x.begin(), x.end(), rz.begin(), T(11));
+  auto e2 = sp::exclusive_scan(sp::par_unseq, x.begin(), x.end(), z.begin(), T(11));
+  CHECK(e2 == z.end(), "exclusive_scan end size %zu", size);
+  CHECK(std::equal(z.begin(), z.end(), rz.begin()), "exclusive_scan size %zu", size);
+  std::exclusive_scan(x.begin(), x.end(), rz.begin(), T(0), acpp::maximum<T>{});
+  sp::exclusive_scan(sp::par_unseq, x.begin(), x.end(), z.begin(), T(0), acpp::maximum<T>{});
+  CHECK(std::equal(z.begin(), z.end(), rz.begin()), "exclusive_scan(max) size %zu", size);
+}
+
+int main() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    std::printf("no GPU\n");
+    return 2;
+  }
+  test_basic_reduction(10, 0);
+  test_basic_reduction(10, 1);
+  test_basic_reduction(10, 127);
+  test_basic_reduction(0, 1000);
+  test_basic_reduction(0ll, 1000 * 1000);
+  std::printf("reduce done\n");
+  for (std::size_t n : {std::size_t(0), std::size_t(1), std::size_t(127), std::size_t(1000), std::size_t(1) << 20}) {
+    test_transform_reduce<int>(n);
+    test_transform_reduce<long long>(n);
+    test_scans<int>(n);
+    test_scans<long long>(n);
+  }
+  std::printf("transform_reduce / scans done\n");
+  std::printf(failures ? "FAILED (%d)\n" : "ALL PASSED\n", failures);
+  return failures ? 1 : 0;
+}

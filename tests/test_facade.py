@@ -71,3 +71,31 @@ def test_facade_on_gpu():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL PASSED" in r.stdout
+
+
+STDPAR_BIN = os.path.join(ROOT, "tests", "lib", "test_stdpar")
+
+
+def test_stdpar_binary_built():
+    assert os.path.exists(STDPAR_BIN), "run __graft_entry__.build() first"
+
+
+def test_stdpar_std_vector_with_custom_allocator_compiles(tmp_path):
+    # std::vector iterators with any allocator are contiguous iterators
+    r = _syntax("""
+#include "acpp/stdpar/numeric.hpp"
+template <class T> struct A : std::allocator<T> {};
+long f(std::vector<long, A<long>>& v, std::vector<long, A<long>>& w) {
+  acpp::stdpar::inclusive_scan(acpp::stdpar::par_unseq, v.begin(), v.end(), w.begin());
+  return acpp::stdpar::reduce(acpp::stdpar::par_unseq, v.cbegin(), v.cend(), 0L);
+}
+""", tmp_path)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_stdpar_on_gpu():
+    r = subprocess.run([STDPAR_BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout
