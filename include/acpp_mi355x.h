@@ -391,6 +391,31 @@ acpp_status acpp_mlp_adam_step(acpp_allocation_group *scratch, acpp_stream_t str
                                const float *target, uint32_t batch, float lr, float weight_decay,
                                float *loss);
 
+/* Configuration query for the segmented forms with a trained model.
+   acpp_cost_model_create copies the parameters (host memory) to `device`
+   with the model's input standardisation (16 means / deviations) and target
+   standardisation (y_mean, y_std of the natural log of microseconds).
+   acpp_cost_model_choose_segmented computes the 16 features of the shape
+   (offsets in HOST memory, num_segments + 1 entries) for each of the 15
+   candidate configurations (work-group 64..1024 x rows 2/4/8), evaluates
+   the network with the forward kernel on `stream`, waits for it, and fills
+   `config` with the candidate of lowest predicted time (*predicted_us, if
+   not NULL).  opensycl_amd/data/cost_model.json holds the model trained on
+   MI355X timings. */
+typedef struct acpp_cost_model acpp_cost_model;
+#define ACPP_ELEM_I32 0
+#define ACPP_ELEM_I64 1
+#define ACPP_ELEM_F32 2
+#define ACPP_ELEM_F64 3
+#define ACPP_SEGMENTED_SCAN 0
+#define ACPP_SEGMENTED_REDUCE 1
+acpp_status acpp_cost_model_create(int device, const float *params, const double *x_mean, const double *x_std,
+                                   double y_mean, double y_std, acpp_cost_model **out);
+acpp_status acpp_cost_model_destroy(acpp_cost_model *model);
+acpp_status acpp_cost_model_choose_segmented(acpp_cost_model *model, acpp_stream_t stream, const uint64_t *offsets,
+                                             uint64_t num_segments, int elem, int kind, acpp_algorithm_op op,
+                                             acpp_launch_config *config, float *predicted_us);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

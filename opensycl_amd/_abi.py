@@ -91,6 +91,10 @@ def _signatures() -> dict[str, tuple[list, object]]:
         ct = SCALAR_TAGS[tag]
         S[f"acpp_copy_if_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_int, ct, ct, vp, cfg], c_int)
         S[f"acpp_copy_if_stencil_{tag}"] = ([vp, vp, vp, c_uint64, vp, vp, vp, cfg], c_int)
+    S["acpp_cost_model_create"] = ([c_int, vp, vp, vp, c_double, c_double, POINTER(vp)], c_int)
+    S["acpp_cost_model_destroy"] = ([vp], c_int)
+    S["acpp_cost_model_choose_segmented"] = ([vp, vp, vp, c_uint64, c_int, c_int, c_int32, POINTER(LaunchConfig),
+                                              POINTER(c_float)], c_int)
     S["acpp_mlp_forward"] = ([vp, vp, vp, c_uint32, vp], c_int)
     S["acpp_mlp_gradient"] = ([vp, vp, vp, vp, vp, c_uint32, vp, vp], c_int)
     S["acpp_mlp_adam_step"] = ([vp, vp, vp, vp, vp, c_uint32, vp, vp, c_uint32, c_float, c_float, vp],

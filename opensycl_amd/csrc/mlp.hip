@@ -20,6 +20,9 @@
 //               inputs give the same bits), optionally applies Adam.
 #include <math.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "internal.hpp"
 #include "launch_util.hpp"
 
@@ -258,6 +261,69 @@ acpp_status train(acpp_allocation_group *g, hipStream_t stream, float *params, f
 }  // namespace mlp
 }  // namespace acpp_impl
 
+// A trained cost model held on one device, with its input / target
+// standardisation, for configuration queries.
+struct acpp_cost_model {
+  int device = 0;
+  float *params = nullptr;  // device, ACPP_MLP_PARAMS
+  float *x = nullptr;       // device, candidates x 16
+  float *y = nullptr;       // device, candidates
+  double x_mean[ACPP_MLP_INPUTS];
+  double x_std[ACPP_MLP_INPUTS];
+  double y_mean = 0, y_std = 1;
+};
+
+namespace acpp_impl {
+namespace mlp {
+
+// Candidate configurations of the segmented forms, in the order the model
+// was trained on: work-group size x rows.
+constexpr uint32_t seg_wgs[] = {64, 128, 256, 512, 1024};
+constexpr uint32_t seg_rows[] = {2, 4, 8};
+constexpr int seg_candidates = 15;
+
+// The 16 features of (shape, configuration): log2(n+1), log2(S+1), mean,
+// deviation, maximum and skew (clipped to +-5) of log2(length+1), fraction
+// of empty segments, log2(element bytes), is-float, is-scan, is-plus;
+// log2(work-group size), log2(rows), log2(tile elements),
+// log2(tiles + 1), log2(S / (tiles + 1) + 1).
+void segmented_features(const uint64_t *offsets, uint64_t S, int elem_bytes, bool is_float, bool is_scan,
+                        bool is_plus, double shape[11]) {
+  const uint64_t n = offsets[S] - offsets[0];
+  double sum = 0, mx = 0, empty = 0;
+  for (uint64_t s = 0; s < S; ++s) {
+    const double len = static_cast<double>(offsets[s + 1] - offsets[s]);
+    const double l = std::log2(len + 1.0);
+    sum += l;
+    mx = std::max(mx, l);
+    empty += (len == 0) ? 1.0 : 0.0;
+  }
+  const double mu = S ? sum / S : 0.0;
+  double m2 = 0, m3 = 0;
+  for (uint64_t s = 0; s < S; ++s) {
+    const double d = std::log2(static_cast<double>(offsets[s + 1] - offsets[s]) + 1.0) - mu;
+    m2 += d * d;
+    m3 += d * d * d;
+  }
+  const double sd = S ? std::sqrt(m2 / S) : 0.0;
+  double skew = (sd > 1e-9) ? (m3 / S) / (sd * sd * sd) : 0.0;
+  skew = std::min(5.0, std::max(-5.0, skew));
+  shape[0] = std::log2(static_cast<double>(n) + 1.0);
+  shape[1] = std::log2(static_cast<double>(S) + 1.0);
+  shape[2] = mu;
+  shape[3] = sd;
+  shape[4] = mx;
+  shape[5] = skew;
+  shape[6] = S ? empty / S : 0.0;
+  shape[7] = std::log2(static_cast<double>(elem_bytes));
+  shape[8] = is_float ? 1.0 : 0.0;
+  shape[9] = is_scan ? 1.0 : 0.0;
+  shape[10] = is_plus ? 1.0 : 0.0;
+}
+
+}  // namespace mlp
+}  // namespace acpp_impl
+
 using namespace acpp_impl;
 
 extern "C" {
@@ -279,6 +345,110 @@ acpp_status acpp_mlp_gradient(acpp_allocation_group *scratch, acpp_stream_t stre
   if (!grad) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "acpp_mlp_gradient: grad is NULL");
   return mlp::train(scratch, static_cast<hipStream_t>(stream), const_cast<float *>(params), nullptr, nullptr, 0,
                     x, target, batch, 0.0f, 0.0f, grad, loss, false, "acpp_mlp_gradient");
+}
+
+acpp_status acpp_cost_model_create(int device, const float *params, const double *x_mean, const double *x_std,
+                                   double y_mean, double y_std, acpp_cost_model **out) {
+  if (!params || !x_mean || !x_std || !out)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, "acpp_cost_model_create: NULL pointer argument");
+  int prev = 0;
+  ACPP_CHECK_HIP(hipGetDevice(&prev));
+  if (prev != device) ACPP_CHECK_HIP(hipSetDevice(device));
+  auto *m = new acpp_cost_model;
+  m->device = device;
+  for (int i = 0; i < ACPP_MLP_INPUTS; ++i) {
+    m->x_mean[i] = x_mean[i];
+    m->x_std[i] = x_std[i] != 0.0 ? x_std[i] : 1.0;
+  }
+  m->y_mean = y_mean;
+  m->y_std = y_std;
+  hipError_t e = hipMalloc(&m->params, ACPP_MLP_PARAMS * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&m->x, mlp::seg_candidates * ACPP_MLP_INPUTS * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&m->y, mlp::seg_candidates * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(m->params, params, ACPP_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice);
+  if (prev != device) (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    (void)hipFree(m->params);
+    (void)hipFree(m->x);
+    (void)hipFree(m->y);
+    delete m;
+    return hip_error(e, "acpp_cost_model_create");
+  }
+  *out = m;
+  return clear_error();
+}
+
+acpp_status acpp_cost_model_destroy(acpp_cost_model *m) {
+  if (!m) return clear_error();
+  (void)hipFree(m->params);
+  (void)hipFree(m->x);
+  (void)hipFree(m->y);
+  delete m;
+  return clear_error();
+}
+
+acpp_status acpp_cost_model_choose_segmented(acpp_cost_model *m, acpp_stream_t stream, const uint64_t *offsets,
+                                             uint64_t num_segments, int elem, int kind, acpp_algorithm_op op,
+                                             acpp_launch_config *config, float *predicted_us) {
+  const char *name = "acpp_cost_model_choose_segmented";
+  if (!m || !offsets || !config) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": NULL pointer argument");
+  if (num_segments == 0) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": no segments");
+  int bytes = 0;
+  bool is_float = false;
+  switch (elem) {
+    case ACPP_ELEM_I32: bytes = 4; break;
+    case ACPP_ELEM_I64: bytes = 8; break;
+    case ACPP_ELEM_F32: bytes = 4; is_float = true; break;
+    case ACPP_ELEM_F64: bytes = 8; is_float = true; break;
+    default: return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": element type must be i32, i64, f32 or f64");
+  }
+  if (kind != ACPP_SEGMENTED_SCAN && kind != ACPP_SEGMENTED_REDUCE)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": kind must be scan or reduce");
+  double shape[11];
+  mlp::segmented_features(offsets, num_segments, bytes, is_float, kind == ACPP_SEGMENTED_SCAN, op == ACPP_OP_PLUS,
+                          shape);
+  const double n = static_cast<double>(offsets[num_segments] - offsets[0]);
+  float x[mlp::seg_candidates * ACPP_MLP_INPUTS];
+  int c = 0;
+  for (uint32_t wg : mlp::seg_wgs)
+    for (uint32_t rows : mlp::seg_rows) {
+      double f[ACPP_MLP_INPUTS];
+      for (int i = 0; i < 11; ++i) f[i] = shape[i];
+      const double tile = static_cast<double>(wg) * rows * (16 / bytes);
+      const double tiles = n / tile;
+      f[11] = std::log2(static_cast<double>(wg));
+      f[12] = std::log2(static_cast<double>(rows));
+      f[13] = std::log2(tile);
+      f[14] = std::log2(tiles + 1.0);
+      f[15] = std::log2(static_cast<double>(num_segments) / (tiles + 1.0) + 1.0);
+      for (int i = 0; i < ACPP_MLP_INPUTS; ++i)
+        x[c * ACPP_MLP_INPUTS + i] = static_cast<float>((f[i] - m->x_mean[i]) / m->x_std[i]);
+      ++c;
+    }
+  int prev = 0;
+  ACPP_CHECK_HIP(hipGetDevice(&prev));
+  if (prev != m->device) ACPP_CHECK_HIP(hipSetDevice(m->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float y[mlp::seg_candidates];
+  hipError_t e = hipMemcpyAsync(m->x, x, sizeof(x), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(mlp::forward_kernel, dim3(1), dim3(256), 0, s, m->params, m->x,
+                       static_cast<uint32_t>(mlp::seg_candidates), m->y);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(y, m->y, sizeof(y), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // x and y live on this stack frame
+  if (prev != m->device) (void)hipSetDevice(prev);
+  if (e != hipSuccess) return hip_error(e, name);
+  int best = 0;
+  for (int i = 1; i < mlp::seg_candidates; ++i)
+    if (y[i] < y[best]) best = i;
+  config->work_group_size = mlp::seg_wgs[best / 3];
+  config->items_per_thread = mlp::seg_rows[best % 3];
+  config->groups_per_cu = 0;
+  config->flags = 0;
+  if (predicted_us) *predicted_us = static_cast<float>(std::exp(y[best] * m->y_std + m->y_mean));
+  return clear_error();
 }
 
 acpp_status acpp_mlp_adam_step(acpp_allocation_group *scratch, acpp_stream_t stream, float *params,

@@ -315,6 +315,51 @@ class LearnedTuner:
         return min(t, key=t.get)
 
 
+class NativeTuner:
+    """The same choice through the C-ABI (acpp_cost_model_choose_segmented):
+    features computed by the library from host offsets, network evaluated
+    by its forward kernel.  What a C or C++ caller uses."""
+
+    _ELEM = {"i32": 0, "i64": 1, "f32": 2, "f64": 3}
+
+    def __init__(self, model: Optional[CostModel] = None, path: str = DEFAULT_MODEL, device: int = 0):
+        import ctypes
+        from . import _abi
+        if model is None:
+            model = CostModel.load(path)
+        self._lib = _abi.load()
+        p = np.ascontiguousarray(model.params, dtype=np.float32)
+        xm = np.ascontiguousarray(model.x_mean, dtype=np.float64)
+        xs = np.ascontiguousarray(model.x_std, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _abi.check(self._lib.acpp_cost_model_create(device, p.ctypes.data, xm.ctypes.data, xs.ctypes.data,
+                                                    model.y_mean, model.y_std, ctypes.byref(h)))
+        self._h = h
+
+    def choose(self, shape: Shape, stream: int = 0):
+        import ctypes
+        from . import _abi
+        off = np.ascontiguousarray(shape.offsets(), dtype=np.uint64)
+        cfg = _abi.LaunchConfig()
+        pred = ctypes.c_float()
+        _abi.check(self._lib.acpp_cost_model_choose_segmented(
+            self._h, stream, off.ctypes.data, shape.num_segments, self._ELEM[shape.dtype],
+            0 if shape.kind == "scan" else 1, 0 if shape.op == "plus" else 3, ctypes.byref(cfg),
+            ctypes.byref(pred)))
+        return (cfg.work_group_size, cfg.items_per_thread), float(pred.value)
+
+    def close(self):
+        if self._h:
+            self._lib.acpp_cost_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def regret(db: list, picks: list) -> dict:
     """Regret t(pick)/t(best) - 1 over the records, with the same statistic
     for the library's default configuration (the method without the cost
