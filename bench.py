@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-segmented", action="store_true", help="skip the segmented leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the precision / atomics configs")
+    ap.add_argument("--no-per-segment", action="store_true",
+                    help="segmented leg without the per-segment baseline (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain elementwise pass over the int32 input (4n bytes read, "
                          "4n written) to calibrate the profiler's traffic counters")
@@ -229,8 +231,8 @@ def main():
 
     if not args.no_segmented:
         try:
-            result["segmented"] = segmented_leg(acpp, q, cache)
-            result["segmented_speedup"] = {k: v["speedup_graph_vs_per_segment"]
+            result["segmented"] = segmented_leg(acpp, q, cache, per_segment=not args.no_per_segment)
+            result["segmented_speedup"] = {k: v.get("speedup_graph_vs_per_segment")
                                            for k, v in result["segmented"].items() if isinstance(v, dict)}
         except Exception as e:  # report, do not hide the headline numbers
             log(f"segmented leg failed: {e!r}")
@@ -255,7 +257,7 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def segmented_leg(acpp, q, cache):
+def segmented_leg(acpp, q, cache, per_segment=True):
     """65,536 log-normal segments: batched (graph plan / direct) against one
     plain call per segment from native code (opensycl_amd/lib/libacpp_bench.so)."""
     import numpy as np
@@ -303,21 +305,25 @@ def segmented_leg(acpp, q, cache):
             g2.release()
             direct_avg, _ = timed(q, direct2, 10, release=g2.release)
 
-        def per_segment():
+        def per_segment_s():
             st = bench_lib.acpp_bench_per_segment_inclusive_scan_i64(
                 cache.handle, 0, q.handle, xs.data_ptr(), yref.data_ptr(), off_host.ctypes.data, S)
             if st != 0:
                 raise RuntimeError(f"per-segment scan failed: {st}")
-        per_segment()
-        per_avg, _ = timed(q, per_segment, 2)
+        if per_segment:
+            per_segment_s()
+            per_avg, _ = timed(q, per_segment_s, 2)
+        else:
+            per_avg = float("nan")
         plan.launch(q)
         torch.cuda.synchronize()
-        exact = bool(torch.equal(ys, yref))
+        exact = bool(torch.equal(ys, yref)) if per_segment else None
         b_scan = 16 * T + 8 * (S + 1)
         out["scan_i64"] = {
             "batched_graph_us": round(graph_avg, 1), "batched_graph_min_us": round(graph_min, 1),
-            "batched_direct_us": round(direct_avg, 1), "per_segment_us": round(per_avg, 1),
-            "speedup_graph_vs_per_segment": round(per_avg / graph_avg, 1),
+            "batched_direct_us": round(direct_avg, 1),
+            "per_segment_us": round(per_avg, 1) if per_segment else None,
+            "speedup_graph_vs_per_segment": round(per_avg / graph_avg, 1) if per_segment else None,
             "algorithmic_bytes": b_scan, "gbs": round(b_scan / (graph_avg * 1e-6) / 1e9, 1),
             "bit_exact_vs_per_segment": exact,
         }
@@ -334,8 +340,11 @@ def segmented_leg(acpp, q, cache):
                 ctypes.c_float(0.0))
             if st != 0:
                 raise RuntimeError(f"per-segment reduce failed: {st}")
-        per_segment_r()
-        per_avg, _ = timed(q, per_segment_r, 2)
+        if per_segment:
+            per_segment_r()
+            per_avg, _ = timed(q, per_segment_r, 2)
+        else:
+            per_avg = float("nan")
         plan.launch(q)
         torch.cuda.synchronize()
         exact64 = torch.zeros(S, dtype=torch.float64, device="cuda")
@@ -345,12 +354,12 @@ def segmented_leg(acpp, q, cache):
         abs64 = torch.zeros(S, dtype=torch.float64, device="cuda")
         abs64.index_add_(0, seg_id, xf.double().abs())
         err = ((rs.double() - exact64).abs() / abs64.clamp_min(1e-30)).max().item()
-        err_ref = ((rref.double() - exact64).abs() / abs64.clamp_min(1e-30)).max().item()
+        err_ref = ((rref.double() - exact64).abs() / abs64.clamp_min(1e-30)).max().item() if per_segment else None
         b_red = 4 * T + 8 * (S + 1) + 4 * S
         out["reduce_f32"] = {
             "batched_graph_us": round(graph_avg, 1), "batched_graph_min_us": round(graph_min, 1),
-            "per_segment_us": round(per_avg, 1),
-            "speedup_graph_vs_per_segment": round(per_avg / graph_avg, 1),
+            "per_segment_us": round(per_avg, 1) if per_segment else None,
+            "speedup_graph_vs_per_segment": round(per_avg / graph_avg, 1) if per_segment else None,
             "algorithmic_bytes": b_red, "gbs": round(b_red / (graph_avg * 1e-6) / 1e9, 1),
             "max_err_over_abs_sum_vs_fp64": err, "per_segment_max_err_over_abs_sum_vs_fp64": err_ref,
         }

@@ -13,9 +13,10 @@
 // counts with popcount, the work-group ranks the selected elements with DPP
 // wave scans and one LDS exchange, one wave looks back over the tile
 // counts (64-bit, one 128-byte status line per tile) for the tile's output
-// offset.  Each row's selected elements are compacted in LDS and stored
-// with consecutive work-items on consecutive output elements (coalesced
-// whatever the selection pattern).  The last tile writes the total count.
+// offset.  The tile's selected elements are compacted in LDS (128 KiB at
+// the default 1024 x 8 tile) and stored with consecutive work-items on
+// consecutive output elements (coalesced whatever the selection pattern).
+// The last tile writes the total count.
 //
 // The predicate is data, not a functor (a functor cannot cross the C-ABI):
 // comparison with a constant, a bit-mask test, or a stencil array for any
@@ -42,6 +43,7 @@ struct params {
   T operand;
   T mask;
   int vec_io;
+  int out_vec;  // output 16-byte aligned
   uint32_t num_tiles;
   uint32_t epoch;
   uint32_t ticket_base;
@@ -49,20 +51,36 @@ struct params {
   uint64_t *status;
 };
 
-template <class T>
+template <int PRED, class T>
 __device__ __forceinline__ bool eval(const params<T> &p, T x, uint64_t idx) {
-  switch (p.pred) {
-    case ACPP_PRED_NONZERO: return x != T(0);
-    case ACPP_PRED_EQ: return x == p.operand;
-    case ACPP_PRED_NE: return x != p.operand;
-    case ACPP_PRED_LT: return x < p.operand;
-    case ACPP_PRED_LE: return x <= p.operand;
-    case ACPP_PRED_GT: return x > p.operand;
-    case ACPP_PRED_GE: return x >= p.operand;
-    case ACPP_PRED_MASK_EQ:
-      if constexpr (std::is_integral_v<T>) return (x & p.mask) == p.operand;
-      return false;
-    default: return p.stencil[idx] != 0;  // ACPP_PRED_STENCIL
+  if constexpr (PRED == ACPP_PRED_NONZERO) return x != T(0);
+  else if constexpr (PRED == ACPP_PRED_EQ) return x == p.operand;
+  else if constexpr (PRED == ACPP_PRED_NE) return x != p.operand;
+  else if constexpr (PRED == ACPP_PRED_LT) return x < p.operand;
+  else if constexpr (PRED == ACPP_PRED_LE) return x <= p.operand;
+  else if constexpr (PRED == ACPP_PRED_GT) return x > p.operand;
+  else if constexpr (PRED == ACPP_PRED_GE) return x >= p.operand;
+  else if constexpr (PRED == ACPP_PRED_MASK_EQ) {
+    if constexpr (std::is_integral_v<T>) return (x & p.mask) == p.operand;
+    else return false;
+  } else {
+    return p.stencil[idx] != 0;  // ACPP_PRED_STENCIL
+  }
+}
+
+// Selection bits of the tile: bit k of fm[r] = element k of row r is copied.
+template <int PRED, class T, int R, int VEC>
+__device__ __forceinline__ void select_bits(const params<T> &p, const T (&v)[R][VEC], uint64_t base, bool full,
+                                            uint32_t (&fm)[R]) {
+  const int B = blockDim.x;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    fm[r] = 0;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      const uint64_t idx = base + static_cast<uint64_t>(r * B + threadIdx.x) * VEC + k;
+      if ((full || idx < p.n) && eval<PRED>(p, v[r][k], idx)) fm[r] |= 1u << k;
+    }
   }
 }
 
@@ -80,7 +98,7 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wave[R][scan_detail::max_waves];
   __shared__ uint64_t s_excl;
-  extern __shared__ unsigned char s_stage_raw[];  // one compacted row: B * VEC elements
+  extern __shared__ unsigned char s_stage_raw[];  // the compacted tile: B * R * VEC elements
   T *s_stage = reinterpret_cast<T *>(s_stage_raw);
 
   if (tid == 0)
@@ -111,16 +129,19 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
         v[r][k] = (idx < p.n) ? p.in[idx] : T(0);
       }
   }
-  // Selection bits: bit k of fm[r] = element k of row r is copied.
+  // Selection bits; one instance of the loop per predicate kind (the
+  // switch is outside the per-element work).
   uint32_t fm[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    fm[r] = 0;
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-      const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
-      if (idx < p.n && eval(p, v[r][k], idx)) fm[r] |= 1u << k;
-    }
+  switch (p.pred) {
+    case ACPP_PRED_NONZERO: select_bits<ACPP_PRED_NONZERO>(p, v, base, full, fm); break;
+    case ACPP_PRED_EQ: select_bits<ACPP_PRED_EQ>(p, v, base, full, fm); break;
+    case ACPP_PRED_NE: select_bits<ACPP_PRED_NE>(p, v, base, full, fm); break;
+    case ACPP_PRED_LT: select_bits<ACPP_PRED_LT>(p, v, base, full, fm); break;
+    case ACPP_PRED_LE: select_bits<ACPP_PRED_LE>(p, v, base, full, fm); break;
+    case ACPP_PRED_GT: select_bits<ACPP_PRED_GT>(p, v, base, full, fm); break;
+    case ACPP_PRED_GE: select_bits<ACPP_PRED_GE>(p, v, base, full, fm); break;
+    case ACPP_PRED_MASK_EQ: select_bits<ACPP_PRED_MASK_EQ>(p, v, base, full, fm); break;
+    default: select_bits<ACPP_PRED_STENCIL>(p, v, base, full, fm); break;
   }
   // Ranks inside the tile.
   uint32_t wexcl[R];
@@ -163,26 +184,42 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
   }
   if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + agg;
 
-  // Row by row: compact the row's selected elements in LDS, then store the
-  // row's output range with consecutive work-items on consecutive elements
-  // (fully coalesced, whatever the selection pattern).
-  uint32_t row_start = 0;
+  // Compact the tile's selected elements in LDS, then store the tile's
+  // output range with consecutive work-items on consecutive elements (fully
+  // coalesced whatever the selection pattern; one barrier per tile).
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    uint32_t row_count = 0;
-    for (int ww = 0; ww < nw; ++ww) row_count += s_wave[r][ww];
-    uint32_t pos = before[r] - row_start + wexcl[r];
+    uint32_t pos = before[r] + wexcl[r];
     const uint32_t bits = fm[r];
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
       if (bits & (1u << k)) s_stage[pos++] = v[r][k];
     }
-    __syncthreads();
-    T *dst = p.out + tile_prefix + row_start;
-    for (uint32_t i = tid; i < row_count; i += B) dst[i] = s_stage[i];
-    __syncthreads();  // the stage is rewritten by the next row
-    row_start += row_count;
   }
+  __syncthreads();
+  T *dst = p.out + tile_prefix;
+  const uint32_t count = static_cast<uint32_t>(agg);
+  if (!p.out_vec) {
+    for (uint32_t i = tid; i < count; i += B) dst[i] = s_stage[i];
+    return;
+  }
+  // 16-byte stores for the part of the range that is 16-byte aligned in the
+  // output; the elements before and after it are stored one by one.
+  const uint32_t mis = static_cast<uint32_t>(tile_prefix % VEC);
+  uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
+  if (head > count) head = count;
+  const uint32_t nvec = (count - head) / VEC;
+  const uint32_t tail = head + nvec * VEC;
+  if (static_cast<uint32_t>(tid) < head) dst[tid] = s_stage[tid];
+  for (uint32_t j = tid; j < nvec; j += B) {
+    T t[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) t[k] = s_stage[head + j * VEC + k];
+    u32x4 q;
+    __builtin_memcpy(&q, t, 16);
+    reinterpret_cast<u32x4 *>(dst + head)[j] = q;
+  }
+  if (static_cast<uint32_t>(tid) < count - tail) dst[tail + tid] = s_stage[tail + tid];
 }
 
 __global__ void store_zero_kernel(uint64_t *p) { *p = 0; }
@@ -259,12 +296,13 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   p.operand = operand;
   p.mask = mask;
   p.vec_io = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  p.out_vec = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.epoch = epoch;
   p.ticket_base = ticket_base;
   p.sync = static_cast<sync_header *>(region);
   p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
-  const size_t stage_bytes = static_cast<size_t>(wg) * VEC * sizeof(T);
+  const size_t stage_bytes = static_cast<size_t>(tile_elems) * sizeof(T);  // 128 KiB at 1024 x 8
   hipLaunchKernelGGL((copy_if_kernel<T, rows<T>()>), dim3(static_cast<uint32_t>(num_tiles)), dim3(wg),
                      stage_bytes, stream, p);
   st = check_launch(name);

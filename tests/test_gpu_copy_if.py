@@ -75,6 +75,21 @@ def test_stencil_and_misaligned(gpu, q, scratch, dtype):
         assert c == rc and np.array_equal(out, ref), offset
 
 
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_unaligned_output(gpu, q, scratch, dtype):
+    # output not 16-byte aligned: element-wise stores
+    torch, _ = gpu
+    x = (np.arange(1_000_003) % 97).astype(dtype)
+    xd = dev(torch, x)
+    d_full = torch.empty(x.size + 4, dtype=xd.dtype, device="cuda")
+    cnt64 = torch.zeros(1, dtype=torch.int64, device="cuda")
+    acpp.copy_if(q, scratch, xd, d_full[1:], acpp.PRED_LT, dtype(40), 0, cnt64.view(torch.uint64))
+    q.wait()
+    ref, rc = O.copy_if(x, O.PRED_LT, dtype(40))
+    assert int(cnt64.item()) == rc
+    assert np.array_equal(host(d_full)[1:1 + rc], ref)
+
+
 @pytest.mark.parametrize("cfg", [(64, 0), (256, 0), (512, 8), (1024, 8)])
 def test_work_group_sizes(gpu, q, scratch, cfg):
     torch, _ = gpu
