@@ -133,14 +133,17 @@ __global__ __launch_bounds__(1024) void histogram_kernel(const uint32_t *__restr
   }
 }
 
-// bins[b] += sum over groups of partials[g][b]
+// bins[b] += sum over groups of partials[g][b].  blockIdx.y takes every
+// gridDim.y-th group, so the partials are read by many work-groups at once;
+// the per-chunk sums are added with device-scope integer atomics (exact, so
+// the order does not matter).
 __global__ void histogram_combine_kernel(const uint32_t *__restrict__ partials, uint32_t groups,
                                          uint32_t num_bins, uint32_t *bins) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= num_bins) return;
   uint32_t s = 0;
-  for (uint32_t g = 0; g < groups; ++g) s += partials[static_cast<uint64_t>(g) * num_bins + b];
-  bins[b] += s;
+  for (uint32_t g = blockIdx.y; g < groups; g += gridDim.y) s += partials[static_cast<uint64_t>(g) * num_bins + b];
+  if (s) __hip_atomic_fetch_add(&bins[b], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // --------------------------------------------------------------------------
@@ -340,8 +343,9 @@ acpp_status histogram(acpp_allocation_group *g, hipStream_t stream, const uint32
                      static_cast<uint32_t *>(partials), bins);
   st = check_launch(name);
   if (st != ACPP_SUCCESS) return st;
-  hipLaunchKernelGGL(histogram_combine_kernel, dim3(static_cast<uint32_t>(ceil_div(num_bins, 256))), dim3(256),
-                     0, stream, static_cast<const uint32_t *>(partials), grid, num_bins, bins);
+  const uint32_t chunks = std::min<uint32_t>(grid, 32u);
+  hipLaunchKernelGGL(histogram_combine_kernel, dim3(static_cast<uint32_t>(ceil_div(num_bins, 256)), chunks),
+                     dim3(256), 0, stream, static_cast<const uint32_t *>(partials), grid, num_bins, bins);
   st = check_launch(name);
   return st == ACPP_SUCCESS ? clear_error() : st;
 }
