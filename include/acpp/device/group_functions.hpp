@@ -46,7 +46,7 @@ struct known_identity {
 #define ACPP_KNOWN_IDENTITY(OP, OPSFN)                                            \
   template <class U, class T>                                                     \
   struct known_identity<OP<U>, T> {                                               \
-    static constexpr bool value = std::is_arithmetic_v<T>;                        \
+    static constexpr bool value = std::is_integral_v<T> || ops::is_floating_v<T>;  \
     __host__ __device__ static T get() { return ops::OPSFN<T, T>::identity(); }   \
   };
 ACPP_KNOWN_IDENTITY(plus, plus)
@@ -557,8 +557,8 @@ __device__ T select_from_group(group g, T x, uint32_t remote_local_id) {
 // (include/hipSYCL/sycl/libkernel/sscp/builtins/reduction.hpp:16-80,
 // scan_inclusive.hpp:16-86, scan_exclusive.hpp:16-102).  The operator is
 // the __acpp_sscp_algorithm_op value (builtin_config.hpp:27-37).
-// Floating-point variants accept plus, multiply, min and max and return T{}
-// for the other operators, like the reference's AMDGPU builtins
+// Floating-point variants (f16 as _Float16, f32, f64) accept plus,
+// multiply, min and max and return T{} for the other operators, like the reference's AMDGPU builtins
 // (src/libkernel/sscp/amdgpu/reduction.cpp:15-31).
 // ---------------------------------------------------------------------------
 namespace acpp {
@@ -630,6 +630,7 @@ ACPP_SSCP_COLLECTIVES(u8, uint8_t)
 ACPP_SSCP_COLLECTIVES(u16, uint16_t)
 ACPP_SSCP_COLLECTIVES(u32, uint32_t)
 ACPP_SSCP_COLLECTIVES(u64, uint64_t)
+ACPP_SSCP_COLLECTIVES(f16, _Float16)
 ACPP_SSCP_COLLECTIVES(f32, float)
 ACPP_SSCP_COLLECTIVES(f64, double)
 #undef ACPP_SSCP_COLLECTIVES

@@ -41,6 +41,26 @@
 namespace acpp {
 namespace ops {
 
+// Floating-point element types: the standard ones plus _Float16 (the
+// half-precision storage type of the f16 builtins), for which the standard
+// library's is_floating_point and numeric_limits say nothing.
+template <class W>
+inline constexpr bool is_floating_v = std::is_floating_point_v<W>
+#if defined(__clang__)
+                                      || std::is_same_v<std::remove_cv_t<W>, _Float16>
+#endif
+    ;
+
+template <class W>
+ACPP_HD W infinity() {
+#if defined(__clang__)
+  if constexpr (std::is_same_v<W, _Float16>)
+    return static_cast<_Float16>(__builtin_huge_valf());
+  else
+#endif
+    return std::numeric_limits<W>::infinity();
+}
+
 enum algorithm_op : int32_t {
   op_plus = 0,
   op_multiply = 1,
@@ -78,7 +98,7 @@ struct plus {
   static constexpr int id = op_plus;
   ACPP_HD static W apply(W a, W b) { return wrap_add(a, b); }
   ACPP_HD static W identity() {
-    if constexpr (std::is_floating_point_v<W>)
+    if constexpr (is_floating_v<W>)
       return W(-0.0);
     else
       return W(0);
@@ -98,8 +118,8 @@ struct minimum {
   static constexpr int id = op_min;
   ACPP_HD static W apply(W a, W b) { return (a < b) ? a : b; }
   ACPP_HD static W identity() {
-    if constexpr (std::is_floating_point_v<W>)
-      return std::numeric_limits<W>::infinity();
+    if constexpr (is_floating_v<W>)
+      return infinity<W>();
     else
       return static_cast<W>(std::numeric_limits<T>::max());
   }
@@ -111,8 +131,8 @@ struct maximum {
   static constexpr int id = op_max;
   ACPP_HD static W apply(W a, W b) { return (a > b) ? a : b; }
   ACPP_HD static W identity() {
-    if constexpr (std::is_floating_point_v<W>)
-      return -std::numeric_limits<W>::infinity();
+    if constexpr (is_floating_v<W>)
+      return -infinity<W>();
     else
       return static_cast<W>(std::numeric_limits<T>::lowest());
   }

@@ -90,6 +90,7 @@ ABI_CALL(i8, int8_t)
 ABI_CALL(i32, int32_t)
 ABI_CALL(i64, int64_t)
 ABI_CALL(u32, uint32_t)
+ABI_CALL(f16, _Float16)
 ABI_CALL(f32, float)
 ABI_CALL(f64, double)
 
@@ -99,6 +100,7 @@ __device__ T abi_dispatch(int kind, int op, T x, T init) {
   else if constexpr (std::is_same_v<T, int32_t>) return abi_i32(kind, op, x, init);
   else if constexpr (std::is_same_v<T, int64_t>) return abi_i64(kind, op, x, init);
   else if constexpr (std::is_same_v<T, uint32_t>) return abi_u32(kind, op, x, init);
+  else if constexpr (std::is_same_v<T, _Float16>) return abi_f16(kind, op, x, init);
   else if constexpr (std::is_same_v<T, float>) return abi_f32(kind, op, x, init);
   else return abi_f64(kind, op, x, init);
 }
@@ -339,7 +341,7 @@ int launch_movement(int level, int which, uint32_t param, const void *in, const 
 
 extern "C" {
 
-// tag: 0 i8, 1 i32, 2 i64, 3 u32, 4 f32, 5 f64
+// tag: 0 i8, 1 i32, 2 i64, 3 u32, 4 f32, 5 f64, 6 f16
 int tk_group_collective(int tag, int kind, const void *in, void *out, uint64_t n,
                         uint32_t local_size, int op, const void *init, void *stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -350,6 +352,7 @@ int tk_group_collective(int tag, int kind, const void *in, void *out, uint64_t n
     case 3: return launch_collective<uint32_t>(kind, in, out, n, local_size, op, init, s);
     case 4: return launch_collective<float>(kind, in, out, n, local_size, op, init, s);
     case 5: return launch_collective<double>(kind, in, out, n, local_size, op, init, s);
+    case 6: return launch_collective<_Float16>(kind, in, out, n, local_size, op, init, s);
     default: return 1;
   }
 }

@@ -24,7 +24,8 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
-TAGS = {np.int8: 0, np.int32: 1, np.int64: 2, np.uint32: 3, np.float32: 4, np.float64: 5}
+TAGS = {np.int8: 0, np.int32: 1, np.int64: 2, np.uint32: 3, np.float32: 4, np.float64: 5,
+        np.float16: 6}
 ATOMIC_TAGS = {np.int32: 1, np.int64: 2, np.uint32: 3, np.float32: 4, np.float64: 5, np.uint64: 6}
 CASES = [(25, 100), (144, 576), (256, 1024), (64, 4096), (1024, 8192), (200, 2000)]
 
@@ -140,6 +141,33 @@ def test_group_collectives(gpu, tk, dtype):
                     assert ok, (dtype, local, op, kind)
                 else:
                     assert np.array_equal(got, exp), (dtype, local, op, kind)
+
+
+def test_group_collectives_f16(gpu, tk):
+    """The f16 builtins (__acpp_sscp_*_group_*_f16) and the templates on
+    _Float16.  Sums use integers of magnitude <= 8, whose partial sums stay
+    below 2048 and are exact in half precision in any order, so they and
+    min/max compare bit-exactly against the float32 oracle rounded to half;
+    products of values near 1 round once per multiplication and compare
+    within 2 * local * eps(half) relative."""
+    torch, _ = gpu
+    eps = float(np.finfo(np.float16).eps)
+    for local, glob in CASES:
+        for op in (0, 1, 2, 3):
+            rng = np.random.default_rng(local * 10 + op)
+            if op == 1:
+                x = (1.0 + rng.standard_normal(glob) * 1e-2).astype(np.float16)
+            else:
+                x = np.clip(np.round(rng.standard_normal(glob) * 3), -8, 8).astype(np.float16)
+            init = np.float16(3)
+            for kind in (0, 1, 2, 3, 4, 5, 6, 7, 10, 11, 12, 13, 14, 15):
+                got = _collective(torch, tk, x, local, kind, op, init).astype(np.float64)
+                exp = _expected(x.astype(np.float32), local, kind, op, np.float32(init))
+                exp = exp.astype(np.float16).astype(np.float64)
+                if op == 1:
+                    assert np.all(np.abs(got - exp) <= 2 * local * eps * np.abs(exp)), (local, kind)
+                else:
+                    assert np.array_equal(got, exp), (local, op, kind)
 
 
 def test_exclusive_scan_known_identity(gpu, tk):
