@@ -2,12 +2,15 @@
 // libacpp_mi355x.so through its C-ABI.
 //
 // Same templates, parameter lists and rules as
-// include/hipSYCL/algorithms/numeric.hpp:170-297 (contract in
+// include/hipSYCL/algorithms/numeric.hpp:170-327 (contract in
 // doc/algorithms.md):
 //   transform_reduce(q, scratch, first1, last1, first2, out, init[, reduce, transform][, deps])
 //   reduce(q, scratch, first, last, out[, init[, op]][, deps])
+//   transform_reduce(q, scratch, first, last, out, init, reduce, transform[, deps])
 //   inclusive_scan(q, scratch, first, last, d_first[, op[, init]][, deps])
 //   exclusive_scan(q, scratch, first, last, d_first, init[, op][, deps])
+//   transform_inclusive_scan(q, scratch, first, last, d_first, op, unary[, init][, deps])
+//   transform_exclusive_scan(q, scratch, first, last, d_first, init, op, unary[, deps])
 // Every call is asynchronous on the queue's stream and returns an event;
 // an empty range returns a complete default event and leaves *out untouched
 // (numeric.hpp:165-169).  `deps` are waited for on the stream.
@@ -22,9 +25,14 @@
 //   - reduction / scan operators: acpp::plus ... acpp::logical_or and the
 //     std:: functors of the same operations (acpp/functional.hpp);
 //     floating-point types take plus, multiplies, minimum and maximum;
-//   - transform_reduce: std::plus / acpp::plus with std::multiplies /
+//   - binary transform_reduce: std::plus / acpp::plus with std::multiplies /
 //     acpp::multiplies (the dot product, numeric.hpp:215-223) for int32,
-//     int64, uint32, uint64, float and double.
+//     int64, uint32, uint64, float and double;
+//   - unary transform_reduce and transform_inclusive_scan /
+//     transform_exclusive_scan (numeric.hpp:193-212, 299-327): any of the
+//     operators above with an element transform from acpp/functional.hpp
+//     (acpp::identity, negate, absolute, square, twice, bit_not, is_zero,
+//     is_nonzero, self_op<Op>; std::negate, std::bit_not).
 // Segmented forms (no counterpart in the reference): segmented_inclusive_scan
 // and segmented_reduce over CSR-style offsets.
 #pragma once
@@ -68,6 +76,26 @@ struct abi {
       return acpp_exclusive_scan_##S(g, s, reinterpret_cast<const AT *>(f), n,                     \
                                      reinterpret_cast<AT *>(d), static_cast<AT>(init), op,         \
                                      nullptr);                                                     \
+    }                                                                                              \
+    static acpp_status transform_reduce(acpp_allocation_group *g, hipStream_t s, const CT *f,     \
+                                        uint64_t n, CT *out, CT init, int op, int t) {             \
+      return acpp_transform_reduce_##S(g, s, reinterpret_cast<const AT *>(f), n,                   \
+                                       reinterpret_cast<AT *>(out), static_cast<AT>(init), op, t,  \
+                                       nullptr);                                                   \
+    }                                                                                              \
+    static acpp_status transform_inclusive_scan(acpp_allocation_group *g, hipStream_t s,          \
+                                                const CT *f, uint64_t n, CT *d, int op, int t,     \
+                                                const CT *init) {                                  \
+      return acpp_transform_inclusive_scan_##S(g, s, reinterpret_cast<const AT *>(f), n,           \
+                                               reinterpret_cast<AT *>(d), op, t,                   \
+                                               reinterpret_cast<const AT *>(init), nullptr);       \
+    }                                                                                              \
+    static acpp_status transform_exclusive_scan(acpp_allocation_group *g, hipStream_t s,          \
+                                                const CT *f, uint64_t n, CT *d, CT init, int op,   \
+                                                int t) {                                           \
+      return acpp_transform_exclusive_scan_##S(g, s, reinterpret_cast<const AT *>(f), n,           \
+                                               reinterpret_cast<AT *>(d), static_cast<AT>(init),   \
+                                               op, t, nullptr);                                    \
     }                                                                                              \
   };
 ACPP_FACADE_ABI(signed char, i8, int8_t)
@@ -151,6 +179,18 @@ constexpr int op_code() {
   return code;
 }
 
+template <class F, class T>
+constexpr int unary_code() {
+  constexpr int code = unary_op_of<std::decay_t<F>>::value;
+  static_assert(code >= 0,
+                "acpp::algorithms: the element transform must be acpp::identity/negate/absolute/"
+                "square/twice/bit_not/is_zero/is_nonzero, std::negate, std::bit_not or "
+                "acpp::self_op<Op>");
+  static_assert(std::is_integral<T>::value || code != 5,
+                "acpp::algorithms: bit_not needs an integer element type");
+  return code;
+}
+
 template <class T>
 constexpr void check_type() {
   static_assert(abi<T>::supported,
@@ -198,6 +238,26 @@ event transform_reduce(queue &q, util::allocation_group &scratch, ForwardIt1 fir
                        const std::vector<event> &deps = {}) {
   return transform_reduce(q, scratch, first1, last1, first2, out, init, std::plus<T>{},
                           std::multiplies<T>{}, deps);
+}
+
+// Unary form (numeric.hpp:193-212): *out = reduce(init, transform(x[i])...).
+template <class ForwardIt, class T, class BinaryReductionOp, class UnaryTransformOp>
+event transform_reduce(queue &q, util::allocation_group &scratch, ForwardIt first, ForwardIt last,
+                       T *out, T init, BinaryReductionOp reduce, UnaryTransformOp transform,
+                       const std::vector<event> &deps = {}) {
+  detail::check_type<T>();
+  static_assert(std::is_same<typename detail::contiguous<ForwardIt>::value, T>::value,
+                "acpp::algorithms::transform_reduce: the input value type must equal T");
+  constexpr int op = detail::op_code<BinaryReductionOp, T>();
+  constexpr int t = detail::unary_code<UnaryTransformOp, T>();
+  (void)reduce;
+  (void)transform;
+  if (first == last) return event{};
+  const uint64_t n = static_cast<uint64_t>(std::distance(first, last));
+  q.depends_on(deps);
+  return detail::finish(q, detail::abi<T>::transform_reduce(scratch.native(), q.get_native(),
+                                                            detail::to_ptr(first), n, out, init, op,
+                                                            t));
 }
 
 template <class ForwardIt, class T, class BinaryOp>
@@ -301,6 +361,62 @@ template <class InputIt, class OutputIt, class T>
 event exclusive_scan(queue &q, util::allocation_group &scratch, InputIt first, InputIt last,
                      OutputIt d_first, T init, const std::vector<event> &deps = {}) {
   return exclusive_scan(q, scratch, first, last, d_first, init, std::plus<>{}, deps);
+}
+
+// scan.hpp:85-132: the scans of transform(x[i]).
+template <class InputIt, class OutputIt, class BinaryOp, class UnaryOp>
+event transform_inclusive_scan(queue &q, util::allocation_group &scratch, InputIt first,
+                               InputIt last, OutputIt d_first, BinaryOp binary_op, UnaryOp unary_op,
+                               const std::vector<event> &deps = {}) {
+  detail::check_scan_types<InputIt, OutputIt>();
+  using V = detail::scan_value_t<InputIt, OutputIt>;
+  constexpr int code = detail::op_code<BinaryOp, V>();
+  constexpr int t = detail::unary_code<UnaryOp, V>();
+  (void)binary_op;
+  (void)unary_op;
+  if (first == last) return event{};
+  const uint64_t n = static_cast<uint64_t>(std::distance(first, last));
+  q.depends_on(deps);
+  return detail::finish(q, detail::abi<V>::transform_inclusive_scan(
+                               scratch.native(), q.get_native(), detail::to_ptr(first), n,
+                               detail::to_ptr(d_first), code, t, nullptr));
+}
+
+template <class InputIt, class OutputIt, class BinaryOp, class UnaryOp, class T>
+event transform_inclusive_scan(queue &q, util::allocation_group &scratch, InputIt first,
+                               InputIt last, OutputIt d_first, BinaryOp binary_op, UnaryOp unary_op,
+                               T init, const std::vector<event> &deps = {}) {
+  detail::check_scan_types<InputIt, OutputIt>();
+  using V = detail::scan_value_t<InputIt, OutputIt>;
+  constexpr int code = detail::op_code<BinaryOp, V>();
+  constexpr int t = detail::unary_code<UnaryOp, V>();
+  (void)binary_op;
+  (void)unary_op;
+  if (first == last) return event{};
+  const uint64_t n = static_cast<uint64_t>(std::distance(first, last));
+  const V init_v = static_cast<V>(init);
+  q.depends_on(deps);
+  return detail::finish(q, detail::abi<V>::transform_inclusive_scan(
+                               scratch.native(), q.get_native(), detail::to_ptr(first), n,
+                               detail::to_ptr(d_first), code, t, &init_v));
+}
+
+template <class InputIt, class OutputIt, class T, class BinaryOp, class UnaryOp>
+event transform_exclusive_scan(queue &q, util::allocation_group &scratch, InputIt first,
+                               InputIt last, OutputIt d_first, T init, BinaryOp binary_op,
+                               UnaryOp unary_op, const std::vector<event> &deps = {}) {
+  detail::check_scan_types<InputIt, OutputIt>();
+  using V = detail::scan_value_t<InputIt, OutputIt>;
+  constexpr int code = detail::op_code<BinaryOp, V>();
+  constexpr int t = detail::unary_code<UnaryOp, V>();
+  (void)binary_op;
+  (void)unary_op;
+  if (first == last) return event{};
+  const uint64_t n = static_cast<uint64_t>(std::distance(first, last));
+  q.depends_on(deps);
+  return detail::finish(q, detail::abi<V>::transform_exclusive_scan(
+                               scratch.native(), q.get_native(), detail::to_ptr(first), n,
+                               detail::to_ptr(d_first), static_cast<V>(init), code, t));
 }
 
 // ------------------------------------------------ segmented (extension)

@@ -1,10 +1,10 @@
 // Standard-parallelism entry points for the numeric algorithms: reduce,
-// transform_reduce, inclusive_scan and exclusive_scan with a par_unseq
-// policy, offloaded to the MI355X kernels through the acpp::algorithms
+// transform_reduce, inclusive_scan, exclusive_scan, transform_inclusive_scan
+// and transform_exclusive_scan with a par_unseq policy, offloaded to the MI355X kernels through the acpp::algorithms
 // facade.
 //
 // Same entry points and offload structure as the reference's stdpar layer
-// (include/hipSYCL/std/stdpar/pstl-impl/numeric.hpp:28-470): a thread-local
+// (include/hipSYCL/std/stdpar/pstl-impl/numeric.hpp:28-580): a thread-local
 // in-order queue and thread-local scratch caches
 // (include/hipSYCL/std/stdpar/detail/sycl_glue.hpp:54-90), device scratch
 // for the algorithm and host scratch for a reduction's result, and the
@@ -74,6 +74,19 @@ T transform_reduce(par_unseq_policy, ForwardIt1 first1, ForwardIt1 last1, Forwar
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
   T *out = out_group.template obtain<T>(1);
   algorithms::transform_reduce(rt.q, scratch, first1, last1, first2, out, init, reduce, transform);
+  rt.q.wait();
+  return *out;
+}
+
+template <class ForwardIt, class T, class BinaryReductionOp, class UnaryTransformOp>
+T transform_reduce(par_unseq_policy, ForwardIt first, ForwardIt last, T init, BinaryReductionOp reduce,
+                   UnaryTransformOp transform) {
+  if (first == last) return init;
+  auto &rt = detail::tls_runtime::get();
+  algorithms::util::allocation_group out_group{&rt.host_cache, rt.q};
+  algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
+  T *out = out_group.template obtain<T>(1);
+  algorithms::transform_reduce(rt.q, scratch, first, last, out, init, reduce, transform);
   rt.q.wait();
   return *out;
 }
@@ -150,6 +163,46 @@ OutputIt exclusive_scan(par_unseq_policy, InputIt first, InputIt last, OutputIt 
 template <class InputIt, class OutputIt, class T>
 OutputIt exclusive_scan(par_unseq_policy p, InputIt first, InputIt last, OutputIt d_first, T init) {
   return exclusive_scan(p, first, last, d_first, init, std::plus<>{});
+}
+
+// pstl-impl/numeric.hpp:471-580.
+template <class InputIt, class OutputIt, class BinaryOp, class UnaryOp>
+OutputIt transform_inclusive_scan(par_unseq_policy, InputIt first, InputIt last, OutputIt d_first,
+                                  BinaryOp binary_op, UnaryOp unary_op) {
+  OutputIt result = d_first;
+  std::advance(result, std::distance(first, last));
+  if (first == last) return result;
+  auto &rt = detail::tls_runtime::get();
+  algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
+  algorithms::transform_inclusive_scan(rt.q, scratch, first, last, d_first, binary_op, unary_op);
+  rt.q.wait();
+  return result;
+}
+
+template <class InputIt, class OutputIt, class BinaryOp, class UnaryOp, class T>
+OutputIt transform_inclusive_scan(par_unseq_policy, InputIt first, InputIt last, OutputIt d_first,
+                                  BinaryOp binary_op, UnaryOp unary_op, T init) {
+  OutputIt result = d_first;
+  std::advance(result, std::distance(first, last));
+  if (first == last) return result;
+  auto &rt = detail::tls_runtime::get();
+  algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
+  algorithms::transform_inclusive_scan(rt.q, scratch, first, last, d_first, binary_op, unary_op, init);
+  rt.q.wait();
+  return result;
+}
+
+template <class InputIt, class OutputIt, class T, class BinaryOp, class UnaryOp>
+OutputIt transform_exclusive_scan(par_unseq_policy, InputIt first, InputIt last, OutputIt d_first,
+                                  T init, BinaryOp binary_op, UnaryOp unary_op) {
+  OutputIt result = d_first;
+  std::advance(result, std::distance(first, last));
+  if (first == last) return result;
+  auto &rt = detail::tls_runtime::get();
+  algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
+  algorithms::transform_exclusive_scan(rt.q, scratch, first, last, d_first, init, binary_op, unary_op);
+  rt.q.wait();
+  return result;
 }
 
 }  // namespace stdpar

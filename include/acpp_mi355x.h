@@ -69,6 +69,24 @@ typedef int32_t acpp_algorithm_op;
 #define ACPP_OP_LOGICAL_AND 7
 #define ACPP_OP_LOGICAL_OR 8
 
+/* ------------------------------------------------------------------------ */
+/* Element transforms of transform_reduce (unary form) and of the transform */
+/* scans.  The reference compiles an arbitrary unary callable into the      */
+/* kernel (numeric.hpp:193-212, 299-327); across a C-ABI the transform is   */
+/* one of these codes.  The result has the element type: integer results    */
+/* wrap modulo 2^bits (abs of the most negative value is itself).           */
+/* ACPP_UNARY_BIT_NOT exists only for integer element types.                */
+/* ------------------------------------------------------------------------ */
+typedef int32_t acpp_unary_op;
+#define ACPP_UNARY_IDENTITY 0   /* x */
+#define ACPP_UNARY_NEGATE 1     /* -x (std::negate) */
+#define ACPP_UNARY_ABS 2        /* |x| */
+#define ACPP_UNARY_SQUARE 3     /* x * x */
+#define ACPP_UNARY_TWICE 4      /* x + x */
+#define ACPP_UNARY_BIT_NOT 5    /* ~x (std::bit_not) */
+#define ACPP_UNARY_IS_ZERO 6    /* x == 0 ? 1 : 0 */
+#define ACPP_UNARY_IS_NONZERO 7 /* x != 0 ? 1 : 0 */
+
 typedef void *acpp_stream_t; /* hipStream_t */
 
 /* ------------------------------------------------------------------------ */
@@ -180,6 +198,19 @@ acpp_status acpp_transform_reduce_dot_bf16(acpp_allocation_group *scratch, acpp_
 acpp_status acpp_transform_reduce_dot_compensated_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first1, uint64_t n, const float *first2, float *out, float init, const acpp_launch_config *config);
 acpp_status acpp_transform_reduce_dot_compensated_bf16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first1, uint64_t n, const uint16_t *first2, float *out, float init, const acpp_launch_config *config);
 
+/* Unary transform_reduce (numeric.hpp:193-212):                           */
+/*   *out = op(init, op over transform(x[i])); n == 0 leaves *out as is.    */
+acpp_status acpp_transform_reduce_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *out, int8_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *out, int16_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *out, int32_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *out, int64_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *out, uint8_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *out, uint16_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *out, uint32_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *out, uint64_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, float init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *out, double init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+
 /* ------------------------------------------------------------------------ */
 /* sycl::reduction over a 1-D range (handler.hpp:799-975 with              */
 /* reduction.hpp:165-236 and reduction_descriptor.hpp:25-54):                */
@@ -231,6 +262,35 @@ acpp_status acpp_exclusive_scan_u32(acpp_allocation_group *scratch, acpp_stream_
 acpp_status acpp_exclusive_scan_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *d_first, uint64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, float init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, double init, acpp_algorithm_op op, const acpp_launch_config *config);
+
+/* ------------------------------------------------------------------------ */
+/* Transform scans (numeric.hpp:299-327; scan/scan.hpp:85-132): the scans   */
+/* above over transform(x[i]) instead of x[i].                              */
+/*   transform_inclusive_scan: d[i] = [init op] t(x[0]) op ... op t(x[i])   */
+/*   transform_exclusive_scan: d[0] = init,                                 */
+/*                             d[i] = init op t(x[0]) op ... op t(x[i-1])   */
+/* ------------------------------------------------------------------------ */
+acpp_status acpp_transform_inclusive_scan_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const int8_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const int16_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const int32_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const int64_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const uint8_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const uint16_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const uint32_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *d_first, acpp_algorithm_op op, acpp_unary_op transform, const uint64_t *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, acpp_algorithm_op op, acpp_unary_op transform, const float *init, const acpp_launch_config *config);
+acpp_status acpp_transform_inclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, acpp_algorithm_op op, acpp_unary_op transform, const double *init, const acpp_launch_config *config);
+
+acpp_status acpp_transform_exclusive_scan_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *d_first, int8_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *d_first, int16_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, int32_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, int64_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *d_first, uint8_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *d_first, uint16_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *d_first, uint32_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *d_first, uint64_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, float init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_exclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, double init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
 
 /* ------------------------------------------------------------------------ */
 /* Bulk atomics.  These replace a user kernel in which work-item i applies  */

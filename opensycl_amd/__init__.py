@@ -14,6 +14,9 @@ from .algorithms import (ALLOCATION_DEVICE, ALLOCATION_HOST, ALLOCATION_SHARED, 
                          transform_reduce)
 from .algorithms import (PRED_EQ, PRED_GE, PRED_GT, PRED_LE, PRED_LT, PRED_MASK_EQ,  # noqa: F401
                          PRED_NE, PRED_NONZERO, copy_if, copy_if_stencil, reduction_multi)
+from .algorithms import (ABS, BIT_NOT, IDENTITY, IS_NONZERO, IS_ZERO, NEGATE,  # noqa: F401
+                         SQUARE, TWICE, transform_exclusive_scan, transform_inclusive_scan,
+                         transform_reduce_unary)
 from .cost_model import CostModel  # noqa: F401
 
 __version__ = "0.1.0"

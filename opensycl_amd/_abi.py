@@ -60,6 +60,12 @@ def _signatures() -> dict[str, tuple[list, object]]:
         S[f"acpp_reduction_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_int32, c_int, cfg], c_int)
         S[f"acpp_inclusive_scan_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_int32, vp, cfg], c_int)
         S[f"acpp_exclusive_scan_{tag}"] = ([vp, vp, vp, c_uint64, vp, ct, c_int32, cfg], c_int)
+    for tag, ct in SCALAR_TAGS.items():
+        S[f"acpp_transform_reduce_{tag}"] = ([vp, vp, vp, c_uint64, vp, ct, c_int32, c_int32, cfg], c_int)
+        S[f"acpp_transform_inclusive_scan_{tag}"] = (
+            [vp, vp, vp, c_uint64, vp, c_int32, c_int32, vp, cfg], c_int)
+        S[f"acpp_transform_exclusive_scan_{tag}"] = (
+            [vp, vp, vp, c_uint64, vp, ct, c_int32, c_int32, cfg], c_int)
     for tag, ct in DOT_TAGS.items():
         S[f"acpp_transform_reduce_dot_{tag}"] = ([vp, vp, vp, c_uint64, vp, vp, ct, cfg], c_int)
     S["acpp_reduce_bf16"] = ([vp, vp, vp, c_uint64, vp, c_float, c_int32, cfg], c_int)

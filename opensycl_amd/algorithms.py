@@ -34,6 +34,22 @@ _OP_NAMES = {
     "logical_and": LOGICAL_AND, "logical_or": LOGICAL_OR,
 }
 
+# Element transform codes (acpp_unary_op): the unary callables of
+# transform_reduce and the transform scans (numeric.hpp:193-212, 299-327).
+IDENTITY = 0
+NEGATE = 1
+ABS = 2
+SQUARE = 3
+TWICE = 4
+BIT_NOT = 5
+IS_ZERO = 6
+IS_NONZERO = 7
+
+_UNARY_NAMES = {
+    "identity": IDENTITY, "negate": NEGATE, "abs": ABS, "square": SQUARE, "twice": TWICE,
+    "bit_not": BIT_NOT, "is_zero": IS_ZERO, "is_nonzero": IS_NONZERO,
+}
+
 ALLOCATION_DEVICE = 0
 ALLOCATION_SHARED = 1
 ALLOCATION_HOST = 2
@@ -62,6 +78,14 @@ def _op(op) -> int:
             raise ValueError(f"unknown operator {op!r}")
         return _OP_NAMES[op]
     return int(op)
+
+
+def _unary(transform) -> int:
+    if isinstance(transform, str):
+        if transform not in _UNARY_NAMES:
+            raise ValueError(f"unknown element transform {transform!r}")
+        return _UNARY_NAMES[transform]
+    return int(transform)
 
 
 def _cfg(config):
@@ -234,6 +258,17 @@ def transform_reduce(q: Queue, scratch: AllocationGroup, x1, x2, out, init=0,
              _cfg(config)))
 
 
+def transform_reduce_unary(q: Queue, scratch: AllocationGroup, x, out, init=0, op=PLUS,
+                           transform=IDENTITY, config=None) -> None:
+    """Unary transform_reduce (numeric.hpp:193-212):
+    out[0] = op(init, fold_op(transform(x[i]))); empty x leaves out as is."""
+    _check_contig(x, out)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_transform_reduce_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), _scalar(tag, init),
+             _op(op), _unary(transform), _cfg(config)))
+
+
 def reduction(q: Queue, scratch: AllocationGroup, x, out, op=PLUS,
               initialize_to_identity: bool = False, config=None) -> None:
     """parallel_for(range{n}, sycl::reduction(out, op[, initialize_to_identity]))."""
@@ -289,6 +324,32 @@ def exclusive_scan(q: Queue, scratch: AllocationGroup, x, d, init=0, op=PLUS,
     fn = getattr(_abi.load(), f"acpp_exclusive_scan_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _scalar(tag, init),
              _op(op), _cfg(config)))
+
+
+def transform_inclusive_scan(q: Queue, scratch: AllocationGroup, x, d, op=PLUS,
+                             transform=IDENTITY, init=None, config=None) -> None:
+    """algorithms::transform_inclusive_scan (numeric.hpp:299-317): the
+    inclusive scan of transform(x[i])."""
+    _check_contig(x, d)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_transform_inclusive_scan_{tag}")
+    init_p = None
+    if init is not None:
+        init_c = _scalar(tag, init)
+        init_p = ctypes.addressof(init_c)
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _op(op), _unary(transform),
+             init_p, _cfg(config)))
+
+
+def transform_exclusive_scan(q: Queue, scratch: AllocationGroup, x, d, init=0, op=PLUS,
+                             transform=IDENTITY, config=None) -> None:
+    """algorithms::transform_exclusive_scan (numeric.hpp:319-327): the
+    exclusive scan of transform(x[i]) starting from init."""
+    _check_contig(x, d)
+    tag = _tag(x.dtype)
+    fn = getattr(_abi.load(), f"acpp_transform_exclusive_scan_{tag}")
+    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _scalar(tag, init),
+             _op(op), _unary(transform), _cfg(config)))
 
 
 # ---------------------------------------------------------------------------

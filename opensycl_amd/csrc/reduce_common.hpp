@@ -23,6 +23,7 @@
 #pragma once
 
 #include "internal.hpp"
+#include "unary.hpp"
 #include "acpp/device/wave.hpp"
 
 namespace acpp_impl {
@@ -135,6 +136,7 @@ struct params {
   OutT *out;
   typename elem_traits<T>::work init;
   int mode;
+  int unary;  // element transform (unary.hpp) of the unary transform_reduce
 };
 
 // Device-scope (cross-XCD coherent) store / load of a group partial.
@@ -169,8 +171,11 @@ __device__ __forceinline__ void unpack(u32x4 q, W (&out)[VEC]) {
   for (int k = 0; k < VEC; ++k) out[k] = elem_traits<T>::to_work(__builtin_bit_cast(T, tmp[k]));
 }
 
-template <class T, class M, class Op, class OutT, int UNROLL, bool DOT, bool NT = false>
+// XF: apply the element transform p.unary to every element before it is
+// combined (unary transform_reduce); never together with DOT.
+template <class T, class M, class Op, class OutT, int UNROLL, bool DOT, bool NT = false, bool XF = false>
 __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
+  static_assert(!(DOT && XF), "a dot product takes no element transform");
   using W = typename elem_traits<T>::work;
   using V = typename M::value;
   constexpr int VEC = 16 / sizeof(T);
@@ -194,6 +199,19 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
       qa[u] = load16<NT>(va, c + u * stride);
       if constexpr (DOT) qb[u] = load16<NT>(vb, c + u * stride);
     }
+    if constexpr (XF) {
+      // One uniform switch for the whole batch.
+      W xt[UNROLL][VEC];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) unpack<T, W, VEC>(qa[u], xt[u]);
+      unary_detail::apply_tile<T>(p.unary, xt);
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc[u] = M::insert(acc[u], xt[u][k]);
+      }
+      continue;
+    }
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
       W xa[VEC];
@@ -212,6 +230,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   for (; c < p.nvec; c += stride) {
     W xa[VEC];
     unpack<T, W, VEC>(load16<NT>(va, c), xa);
+    if constexpr (XF) unary_detail::apply_row<T>(p.unary, xa);
     if constexpr (DOT) {
       W xb[VEC];
       unpack<T, W, VEC>(load16<NT>(vb, c), xb);
@@ -226,6 +245,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   for (uint64_t s = gtid; s < p.nscalar; s += stride) {
     const uint64_t idx = (s < p.head) ? s : p.tail_start + (s - p.head);
     W xa = elem_traits<T>::to_work(p.a[idx]);
+    if constexpr (XF) xa = unary_detail::apply_one<T>(p.unary, xa);
     if constexpr (DOT) {
       acc[0] = M::insert2(acc[0], xa, elem_traits<T>::to_work(p.b[idx]));
     } else {

@@ -13,11 +13,12 @@ constexpr int vec_width() {
   return 16 / static_cast<int>(sizeof(typename elem_traits<T>::storage));
 }
 
-template <class T, class M, class Op, class OutT, bool DOT>
+// XF: unary transform_reduce with element transform `unary` (unary.hpp).
+template <class T, class M, class Op, class OutT, bool DOT, bool XF = false>
 acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
                    const T *b, uint64_t n, OutT *out,
                    typename elem_traits<T>::work init, int mode,
-                   const acpp_launch_config *cfg, const char *name) {
+                   const acpp_launch_config *cfg, const char *name, int unary = 0) {
   using V = typename M::value;
   constexpr int VEC = vec_width<T>();
   int cus = 0;
@@ -31,7 +32,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   // non-temporal loads once the input is larger than the 256 MiB
   // last-level cache.  The non-temporal choice is the library's only when
   // the whole config is left at zero.
-  constexpr bool tuned = (Op::id == ops::op_plus);
+  constexpr bool tuned = (Op::id == ops::op_plus) && !XF;
   const bool cfg_given = cfg && (cfg->work_group_size || cfg->items_per_thread || cfg->groups_per_cu ||
                                  cfg->flags);
   const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : (tuned ? (DOT ? 1024 : 512) : 256);
@@ -46,6 +47,9 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": items_per_thread must be 2, 4, 8 or 16 (16 only "
                                          "for single-operand reductions)");
+  if (XF && unroll != 4)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT,
+                     std::string(name) + ": items_per_thread must be 4 with an element transform");
 
   // 16-byte aligned region; a dot product needs both operands equally
   // misaligned, otherwise everything goes through the scalar loop.
@@ -68,6 +72,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   p.out = out;
   p.init = init;
   p.mode = mode;
+  p.unary = unary;
 
   auto run = [&](auto kernel) -> acpp_status {
     int resident = 1;
@@ -109,7 +114,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     if (unroll == 8) return run(reduce_kernel<T, M, Op, OutT, 8, DOT>);
     if (unroll == 16) return run(reduce_kernel<T, M, Op, OutT, 16, DOT>);
   }
-  return run(reduce_kernel<T, M, Op, OutT, 4, DOT>);
+  return run(reduce_kernel<T, M, Op, OutT, 4, DOT, false, XF>);
 }
 
 // algorithms::reduce / sycl::reduction entry for element type T.
@@ -167,6 +172,37 @@ acpp_status dot_entry(acpp_allocation_group *g, hipStream_t stream, const T *a, 
   acpp_status s = launch<T, plain_monoid<W, P>, P, OutT, true>(g, stream, a, b, n, out, init,
                                                                final_with_init, cfg, name);
   return s == ACPP_SUCCESS ? clear_error() : s;
+}
+
+// Unary transform_reduce (numeric.hpp:193-212): *out = op(init, op over
+// unary(x[i])); an empty input leaves *out untouched (numeric.hpp:165-169).
+template <class T>
+acpp_status transform_reduce_entry(acpp_allocation_group *g, hipStream_t stream, const T *first,
+                                   uint64_t n, T *out, T init, int op, int unary,
+                                   const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": out is NULL");
+  if (n > 0 && !first) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": first is NULL");
+  if (!unary_detail::valid_code<T>(unary))
+    return set_error(std::is_integral_v<T> || unary < 0 || unary >= unary_detail::u_count
+                         ? ACPP_ERROR_INVALID_ARGUMENT
+                         : ACPP_ERROR_UNSUPPORTED,
+                     std::string(name) + ": unknown transform for this element type");
+  return with_op<W, T>(op, [&](auto opv) -> acpp_status {
+    using Op = decltype(opv);
+    if (n == 0) return clear_error();
+    acpp_status s;
+    if (unary == unary_detail::u_identity)
+      s = launch<T, plain_monoid<W, Op>, Op, T, false>(g, stream, first, nullptr, n, out,
+                                                       elem_traits<T>::to_work(init), final_with_init,
+                                                       cfg, name);
+    else
+      s = launch<T, plain_monoid<W, Op>, Op, T, false, true>(g, stream, first, nullptr, n, out,
+                                                             elem_traits<T>::to_work(init),
+                                                             final_with_init, cfg, name, unary);
+    return s == ACPP_SUCCESS ? clear_error() : s;
+  });
 }
 
 }  // namespace reduce_detail

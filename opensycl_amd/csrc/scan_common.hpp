@@ -41,6 +41,7 @@
 #pragma once
 
 #include "internal.hpp"
+#include "unary.hpp"
 
 namespace acpp_impl {
 namespace scan_detail {
@@ -236,6 +237,7 @@ struct params {
   int exclusive;
   int vec_io;
   int block_lookback;
+  int unary;             // element transform (unary.hpp) of the transform scans
   uint32_t num_tiles;
   uint32_t epoch;        // tag of this launch's status words (>= 1)
   uint32_t ticket_base;  // tile ticket counter value when this launch starts
@@ -243,7 +245,9 @@ struct params {
   uint64_t *status;
 };
 
-template <class T, class Op, int R>
+// XF: apply the element transform p.unary to every input element (the
+// transform scans); false for the plain scans.
+template <class T, class Op, int R, bool XF = false>
 __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   using W = typename elem_traits<T>::work;
   using SIO = status_io<W>;
@@ -293,13 +297,19 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
 #pragma unroll
       for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
     }
+    if constexpr (XF) unary_detail::apply_tile<T>(p.unary, v);
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int k = 0; k < VEC; ++k) {
         const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
-        v[r][k] = (idx < p.n) ? elem_traits<T>::to_work(p.in[idx]) : Op::identity();
+        W x = Op::identity();
+        if (idx < p.n) {
+          x = elem_traits<T>::to_work(p.in[idx]);
+          if constexpr (XF) x = unary_detail::apply_one<T>(p.unary, x);
+        }
+        v[r][k] = x;
       }
     }
   }

@@ -14,10 +14,11 @@ constexpr int default_rows() {
   return sizeof(T) >= 4 ? 8 : (sizeof(T) == 2 ? 4 : 2);
 }
 
-template <class T, class Op>
+// XF: the transform scans (element transform `unary`, see unary.hpp).
+template <class T, class Op, bool XF = false>
 acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, uint64_t n,
                    T *out, bool exclusive, bool has_init, typename elem_traits<T>::work init,
-                   const acpp_launch_config *cfg, const char *name) {
+                   const acpp_launch_config *cfg, const char *name, int unary = 0) {
   using W = typename elem_traits<T>::work;
   constexpr int VEC = 16 / sizeof(T);
   int cus = 0;
@@ -38,7 +39,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
   int rows = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows<T>();
-  constexpr bool tunable = (Op::id == ops::op_plus) && sizeof(T) >= 4;
+  constexpr bool tunable = (Op::id == ops::op_plus) && sizeof(T) >= 4 && !XF;
   if (rows != default_rows<T>()) {
     if (!tunable || (rows != 2 && rows != 4 && rows != 8))
       return set_error(ACPP_ERROR_INVALID_ARGUMENT,
@@ -93,6 +94,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   p.exclusive = exclusive ? 1 : 0;
   p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
   p.block_lookback = (cfg && (cfg->flags & variant_block_lookback)) ? 1 : 0;
+  p.unary = unary;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.epoch = epoch;
   p.ticket_base = ticket_base;
@@ -115,7 +117,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     if (rows == 2) return run(scan_kernel<T, Op, 2>);
     if (rows == 4) return run(scan_kernel<T, Op, 4>);
   }
-  return run(scan_kernel<T, Op, default_rows<T>()>);
+  return run(scan_kernel<T, Op, default_rows<T>(), XF>);
 }
 
 template <class T>
@@ -134,6 +136,40 @@ acpp_status scan_entry(acpp_allocation_group *g, hipStream_t stream, const T *fi
     if (n == 0) return clear_error();
     acpp_status s = launch<T, Op>(g, stream, first, n, d_first, exclusive, has_init,
                                   elem_traits<T>::to_work(init), cfg, name);
+    return s == ACPP_SUCCESS ? clear_error() : s;
+  });
+}
+
+// transform_inclusive_scan / transform_exclusive_scan (numeric.hpp:299-327):
+// the scan of unary(x[i]) for a transform code of unary.hpp.
+template <class T>
+acpp_status transform_scan_entry(acpp_allocation_group *g, hipStream_t stream, const T *first,
+                                 uint64_t n, T *d_first, int op, int unary, bool exclusive,
+                                 const T *init_ptr, T init_value, const acpp_launch_config *cfg,
+                                 const char *name) {
+  using W = typename elem_traits<T>::work;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (n > 0 && (!first || !d_first))
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": first or d_first is NULL");
+  if (!unary_detail::valid_code<T>(unary))
+    return set_error(std::is_integral_v<T> || unary < 0 || unary >= unary_detail::u_count
+                         ? ACPP_ERROR_INVALID_ARGUMENT
+                         : ACPP_ERROR_UNSUPPORTED,
+                     std::string(name) + ": unknown transform for this element type");
+  bool has_init = exclusive || init_ptr != nullptr;
+  T init = exclusive ? init_value : (init_ptr ? *init_ptr : T{});
+  return with_op<W, T>(op, [&](auto opv) -> acpp_status {
+    using Op = decltype(opv);
+    // scan.hpp:85-132 on decoupled_lookback_scan.hpp:565-566: an empty
+    // problem enqueues nothing.
+    if (n == 0) return clear_error();
+    acpp_status s;
+    if (unary == unary_detail::u_identity)
+      s = launch<T, Op>(g, stream, first, n, d_first, exclusive, has_init, elem_traits<T>::to_work(init),
+                        cfg, name);
+    else
+      s = launch<T, Op, true>(g, stream, first, n, d_first, exclusive, has_init,
+                              elem_traits<T>::to_work(init), cfg, name, unary);
     return s == ACPP_SUCCESS ? clear_error() : s;
   });
 }

@@ -303,3 +303,32 @@ def copy_if(x: np.ndarray, pred: int, operand=0, mask=0, stencil: np.ndarray | N
     _ok(fn(c_void_p(_p(x)), c_uint64(x.size), c_void_p(_p(st) if st is not None else 0), c_int(pred),
            ct(operand), ct(mask), c_void_p(_p(out)), ctypes.byref(cnt)))
     return out[:cnt.value].copy(), int(cnt.value)
+
+
+# element transform codes (== acpp_unary_op)
+IDENTITY, NEGATE, ABS, SQUARE, TWICE, BIT_NOT, IS_ZERO, IS_NONZERO = range(8)
+
+
+def transform(x: np.ndarray, code: int) -> np.ndarray:
+    """y[i] = t(x[i]) for a transform code; integer results wrap."""
+    x = np.ascontiguousarray(x)
+    t = _tag(x)
+    y = np.empty_like(x)
+    _ok(getattr(load(), f"oracle_transform_{t}")(c_void_p(_p(x)), c_uint64(x.size),
+                                                 c_void_p(_p(y)), c_int(code)))
+    return y
+
+
+def transform_reduce(x: np.ndarray, init, op: int, code: int):
+    """Unary transform_reduce (numeric.hpp:193-212): reduce over t(x[i])."""
+    return reduce(transform(x, code), init, op)
+
+
+def transform_inclusive_scan(x: np.ndarray, op: int, code: int, init=None) -> np.ndarray:
+    """transform_inclusive_scan (numeric.hpp:299-317; scan.hpp:85-132)."""
+    return inclusive_scan(transform(x, code), op, init)
+
+
+def transform_exclusive_scan(x: np.ndarray, init, op: int, code: int) -> np.ndarray:
+    """transform_exclusive_scan (numeric.hpp:319-327; scan.hpp:85-132)."""
+    return exclusive_scan(transform(x, code), init, op)

@@ -650,3 +650,64 @@ DEF_COPY_IF(i64, int64_t, 1)
 DEF_COPY_IF(u64, uint64_t, 1)
 DEF_COPY_IF(f32, float, 0)
 DEF_COPY_IF(f64, double, 0)
+
+/* ------------------------------------------------------------------------- */
+/* Element transforms of transform_reduce (numeric.hpp:193-212) and of the   */
+/* transform scans (numeric.hpp:299-327; scan/scan.hpp:85-132, whose          */
+/* generator yields unary_op(*it) and then scans as usual).  y[i] =           */
+/* t(x[i]) for the transform codes 0 identity, 1 -x, 2 |x|, 3 x*x, 4 x+x,     */
+/* 5 ~x (integers), 6 x == 0, 7 x != 0.  Integer results wrap modulo          */
+/* 2^bits (computed in the unsigned type WU).  Fault 7 turns |x| into x.      */
+/* ------------------------------------------------------------------------- */
+#define DEF_TRANSFORM_INT(t, T, WU, SIGNED)                                          \
+  int oracle_transform_##t(const T *x, uint64_t n, T *y, int code) {                 \
+    if (code < 0 || code > 7) return -1;                                             \
+    for (uint64_t i = 0; i < n; ++i) {                                               \
+      const T v = x[i];                                                              \
+      const WU u = (WU)v;                                                            \
+      T r = v;                                                                       \
+      switch (code) {                                                                \
+        case 1: r = (T)((WU)0 - u); break;                                           \
+        case 2: r = (SIGNED && ((u >> (8 * sizeof(T) - 1)) & 1u) && g_fault != 7)    \
+                        ? (T)((WU)0 - u) : v;                                        \
+                break;                                                               \
+        case 3: r = (T)(u * u); break;                                               \
+        case 4: r = (T)(u + u); break;                                               \
+        case 5: r = (T)(~u); break;                                                  \
+        case 6: r = (T)(v == (T)0); break;                                           \
+        case 7: r = (T)(v != (T)0); break;                                           \
+        default: break;                                                              \
+      }                                                                              \
+      y[i] = r;                                                                      \
+    }                                                                                \
+    return 0;                                                                        \
+  }
+#define DEF_TRANSFORM_FP(t, T, FABS)                                                 \
+  int oracle_transform_##t(const T *x, uint64_t n, T *y, int code) {                 \
+    if (code < 0 || code > 7 || code == 5) return -1;                                \
+    for (uint64_t i = 0; i < n; ++i) {                                               \
+      const T v = x[i];                                                              \
+      T r = v;                                                                       \
+      switch (code) {                                                                \
+        case 1: r = -v; break;                                                       \
+        case 2: r = (g_fault == 7) ? v : FABS(v); break;                             \
+        case 3: r = v * v; break;                                                    \
+        case 4: r = v + v; break;                                                    \
+        case 6: r = (v == (T)0) ? (T)1 : (T)0; break;                                \
+        case 7: r = (v != (T)0) ? (T)1 : (T)0; break;                                \
+        default: break;                                                              \
+      }                                                                              \
+      y[i] = r;                                                                      \
+    }                                                                                \
+    return 0;                                                                        \
+  }
+DEF_TRANSFORM_INT(i8, int8_t, uint32_t, 1)
+DEF_TRANSFORM_INT(i16, int16_t, uint32_t, 1)
+DEF_TRANSFORM_INT(i32, int32_t, uint32_t, 1)
+DEF_TRANSFORM_INT(i64, int64_t, uint64_t, 1)
+DEF_TRANSFORM_INT(u8, uint8_t, uint32_t, 0)
+DEF_TRANSFORM_INT(u16, uint16_t, uint32_t, 0)
+DEF_TRANSFORM_INT(u32, uint32_t, uint32_t, 0)
+DEF_TRANSFORM_INT(u64, uint64_t, uint64_t, 0)
+DEF_TRANSFORM_FP(f32, float, fabsf)
+DEF_TRANSFORM_FP(f64, double, fabs)

@@ -100,6 +100,19 @@ int oracle_max_threads(void);
    checks catch plausible errors.  0 = faithful (default). */
 void oracle_set_fault(int fault);
 
+/* Element transforms (transform codes 0..7 of acpp_unary_op); -1 for an
+   unknown code or bit_not on floating point. */
+int oracle_transform_i8(const int8_t *x, uint64_t n, int8_t *y, int code);
+int oracle_transform_i16(const int16_t *x, uint64_t n, int16_t *y, int code);
+int oracle_transform_i32(const int32_t *x, uint64_t n, int32_t *y, int code);
+int oracle_transform_i64(const int64_t *x, uint64_t n, int64_t *y, int code);
+int oracle_transform_u8(const uint8_t *x, uint64_t n, uint8_t *y, int code);
+int oracle_transform_u16(const uint16_t *x, uint64_t n, uint16_t *y, int code);
+int oracle_transform_u32(const uint32_t *x, uint64_t n, uint32_t *y, int code);
+int oracle_transform_u64(const uint64_t *x, uint64_t n, uint64_t *y, int code);
+int oracle_transform_f32(const float *x, uint64_t n, float *y, int code);
+int oracle_transform_f64(const double *x, uint64_t n, double *y, int code);
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,6 +169,39 @@ static void test_segmented(acpp::queue &q, alg::util::allocation_cache &cache) {
   std::printf("segmented done\n");
 }
 
+// Unary transform_reduce and the transform scans (numeric.hpp:193-212,
+// 299-327) against std::transform_reduce / std::transform_*_scan.
+static void test_transforms(acpp::queue &q, alg::util::allocation_cache &cache) {
+  for (size_t n : {size_t(1), size_t(1000), size_t(123457)}) {
+    alg::util::allocation_group scratch{&cache, q};
+    auto h = random_ints<int>(n, 7 + n, -1000, 1000);
+    device_buffer<int> x(n), y(n), out(1);
+    x.upload(h);
+    alg::transform_reduce(q, scratch, x.ptr, x.ptr + n, out.ptr, 5, std::plus<int>{}, acpp::square<int>{});
+    q.wait();
+    int ref = std::transform_reduce(h.begin(), h.end(), 5, std::plus<>{}, [](int v) { return v * v; });
+    CHECK(out.download()[0] == ref, "transform_reduce(plus, square) n=%zu", n);
+    alg::transform_reduce(q, scratch, x.ptr, x.ptr + n, out.ptr, 0, acpp::minimum<int>{}, acpp::absolute<int>{});
+    q.wait();
+    ref = std::transform_reduce(h.begin(), h.end(), 0, acpp::minimum<int>{}, [](int v) { return v < 0 ? -v : v; });
+    CHECK(out.download()[0] == ref, "transform_reduce(min, abs) n=%zu", n);
+    std::vector<int> r(n);
+    alg::transform_inclusive_scan(q, scratch, x.ptr, x.ptr + n, y.ptr, std::plus<>{}, std::negate<int>{});
+    q.wait();
+    std::transform_inclusive_scan(h.begin(), h.end(), r.begin(), std::plus<>{}, std::negate<int>{});
+    CHECK(y.download() == r, "transform_inclusive_scan(negate) n=%zu", n);
+    alg::transform_inclusive_scan(q, scratch, x.ptr, x.ptr + n, y.ptr, acpp::maximum<int>{}, acpp::twice<int>{}, -7);
+    q.wait();
+    std::transform_inclusive_scan(h.begin(), h.end(), r.begin(), acpp::maximum<int>{}, [](int v) { return v + v; }, -7);
+    CHECK(y.download() == r, "transform_inclusive_scan(max, twice, init) n=%zu", n);
+    alg::transform_exclusive_scan(q, scratch, x.ptr, x.ptr + n, y.ptr, 3, std::plus<>{}, acpp::is_zero<int>{});
+    q.wait();
+    std::transform_exclusive_scan(h.begin(), h.end(), r.begin(), 3, std::plus<>{}, [](int v) { return v == 0 ? 1 : 0; });
+    CHECK(y.download() == r, "transform_exclusive_scan(is_zero) n=%zu", n);
+  }
+  std::printf("transforms done\n");
+}
+
 static void test_errors(acpp::queue &q, alg::util::allocation_cache &cache) {
   // Misuse reported by the library surfaces as acpp::exception.
   alg::util::allocation_group scratch{&cache, q};
@@ -195,6 +228,7 @@ int main() {
   test_ops_and_init(q, cache);
   test_float_and_dot(q, cache);
   test_segmented(q, cache);
+  test_transforms(q, cache);
   test_errors(q, cache);
   q.wait();
   std::printf(failures ? "FAILED (%d)\n" : "ALL PASSED\n", failures);

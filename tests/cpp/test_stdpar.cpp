@@ -2,7 +2,8 @@
 // in device-accessible (pinned host) memory, checked against the sequential
 // C++ standard library, with the cases of the reference's pstl tests:
 //   tests/pstl/reduce.cpp:24-66 (sizes 0, 1, 127, 1000, 1000*1000; init 10 / 0 / 0ll),
-//   tests/pstl/transform_reduce.cpp, inclusive_scan.cpp, exclusive_scan.cpp.
+//   tests/pstl/transform_reduce.cpp, inclusive_scan.cpp, exclusive_scan.cpp,
+//   transform_inclusive_scan.cpp, transform_exclusive_scan.cpp.
 // Exit status 0 = all passed.
 #include <cstdio>
 #include <numeric>
@@ -92,6 +93,51 @@ void test_scans(std::size_t size) {
   CHECK(std::equal(z.begin(), z.end(), rz.begin()), "exclusive_scan(max) size %zu", size);
 }
 
+// tests/pstl/transform_inclusive_scan.cpp:73-100 and
+// transform_exclusive_scan.cpp:73-93: data[i] = i, unary_op = op(x, x),
+// (plus, init 3) and (multiplies, init 3ull); here unary_op is
+// acpp::self_op<Op>, the same function as a named type.
+template <class T, class Op>
+void test_transform_scans(T init, Op op, std::size_t size) {
+  pvec<T> data(size), d0(size), d1(size), d2(size);
+  for (std::size_t i = 0; i < size; ++i) data[i] = static_cast<T>(i);
+  acpp::self_op<Op> unary{op};
+  std::vector<T> r0(size), r1(size), r2(size);
+  std::transform_inclusive_scan(data.begin(), data.end(), r0.begin(), op, unary);
+  std::transform_inclusive_scan(data.begin(), data.end(), r1.begin(), op, unary, init);
+  std::transform_exclusive_scan(data.begin(), data.end(), r2.begin(), init, op, unary);
+  CHECK(sp::transform_inclusive_scan(sp::par_unseq, data.begin(), data.end(), d0.begin(), op, unary) ==
+            d0.end(),
+        "transform_inclusive_scan end size %zu", size);
+  CHECK(sp::transform_inclusive_scan(sp::par_unseq, data.begin(), data.end(), d1.begin(), op, unary,
+                                     init) == d1.end(),
+        "transform_inclusive_scan(init) end size %zu", size);
+  CHECK(sp::transform_exclusive_scan(sp::par_unseq, data.begin(), data.end(), d2.begin(), init, op,
+                                     unary) == d2.end(),
+        "transform_exclusive_scan end size %zu", size);
+  CHECK(std::equal(d0.begin(), d0.end(), r0.begin()), "transform_inclusive_scan size %zu", size);
+  CHECK(std::equal(d1.begin(), d1.end(), r1.begin()), "transform_inclusive_scan(init) size %zu", size);
+  CHECK(std::equal(d2.begin(), d2.end(), r2.begin()), "transform_exclusive_scan size %zu", size);
+}
+
+// tests/pstl/transform_reduce.cpp:23-35: identity transform, plus, init 10 / 0 / 0ll;
+// plus a negate / square transform on the same data.
+template <class T>
+void test_unary_transform_reduce(T init, std::size_t size) {
+  pvec<T> data(size);
+  for (std::size_t i = 0; i < size; ++i) data[i] = static_cast<T>(i);
+  T ref = std::transform_reduce(data.begin(), data.end(), init, std::plus<>{}, [](auto x) { return x; });
+  T res = sp::transform_reduce(sp::par_unseq, data.begin(), data.end(), init, std::plus<>{}, acpp::identity<>{});
+  CHECK(res == ref, "transform_reduce(identity) size %zu", size);
+  ref = std::transform_reduce(data.begin(), data.end(), init, std::plus<>{}, std::negate<T>{});
+  res = sp::transform_reduce(sp::par_unseq, data.begin(), data.end(), init, std::plus<>{}, std::negate<T>{});
+  CHECK(res == ref, "transform_reduce(negate) size %zu", size);
+  ref = std::transform_reduce(data.begin(), data.end(), init, acpp::maximum<T>{}, acpp::square<T>{});
+  res = sp::transform_reduce(sp::par_unseq, data.begin(), data.end(), init, acpp::maximum<T>{},
+                             acpp::square<T>{});
+  CHECK(res == ref, "transform_reduce(max, square) size %zu", size);
+}
+
 int main() {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
@@ -111,6 +157,17 @@ int main() {
     test_scans<long long>(n);
   }
   std::printf("transform_reduce / scans done\n");
+  test_unary_transform_reduce(10, 0);
+  test_unary_transform_reduce(10, 1);
+  test_unary_transform_reduce(10, 127);
+  test_unary_transform_reduce(0, 1000);
+  test_unary_transform_reduce(0ll, 1000 * 1000);
+  for (std::size_t n : {std::size_t(0), std::size_t(1), std::size_t(127), std::size_t(1000), std::size_t(1024) * 1024}) {
+    test_transform_scans(3u, std::plus<>{}, n);
+    test_transform_scans(3ull, std::multiplies<>{}, n);
+    test_transform_scans(static_cast<std::size_t>(3), std::plus<>{}, n);
+  }
+  std::printf("transform scans done\n");
   std::printf(failures ? "FAILED (%d)\n" : "ALL PASSED\n", failures);
   return failures ? 1 : 0;
 }

@@ -41,6 +41,12 @@ def test_facade_signatures_compile(tmp_path):
   alg::inclusive_scan(q, g, x, x + 10, y, acpp::plus<int>{}, 7, deps);
   alg::exclusive_scan(q, g, x, x + 10, y, 0);
   alg::exclusive_scan(q, g, lx, lx + 10, lx, 0L, acpp::maximum<long>{});
+  alg::transform_reduce(q, g, x, x + 10, y, 0, std::plus<int>{}, acpp::square<int>{});
+  alg::transform_reduce(q, g, fx, fx + 10, fx, 0.f, acpp::maximum<float>{}, acpp::absolute<>{}, deps);
+  alg::transform_inclusive_scan(q, g, x, x + 10, y, std::plus<>{}, std::negate<int>{});
+  alg::transform_inclusive_scan(q, g, x, x + 10, y, std::plus<>{}, acpp::self_op<std::plus<>>{}, 3);
+  alg::transform_exclusive_scan(q, g, lx, lx + 10, lx, 3L, acpp::bit_or<long>{}, std::bit_not<long>{});
+  alg::transform_exclusive_scan(q, g, x, x + 10, y, 0, std::plus<>{}, acpp::is_nonzero<>{}, deps);
 }
 """, tmp_path)
     assert r.returncode == 0, r.stderr
@@ -53,6 +59,12 @@ def test_facade_signatures_compile(tmp_path):
     ("alg::transform_reduce(q, g, x, x + 10, x, y, 0, acpp::maximum<int>{}, std::multiplies<int>{});",
      "reduce = plus with"),
     ("alg::reduce(q, g, x, x + 10, lx, 0L);", "input value type must equal T"),
+    ("alg::transform_reduce(q, g, x, x + 10, y, 0, std::plus<int>{}, [](int v) { return v; });",
+     "element transform must be"),
+    ("alg::transform_inclusive_scan(q, g, fx, fx + 10, fx, std::plus<>{}, acpp::bit_not<>{});",
+     "bit_not needs an integer"),
+    ("alg::transform_exclusive_scan(q, g, x, x + 10, y, 0, std::plus<>{}, acpp::self_op<std::bit_xor<>>{});",
+     "element transform must be"),
 ])
 def test_facade_rejects_at_compile_time(tmp_path, call, message):
     # No generic-functor fallback: what the library lacks fails to compile.

@@ -1,0 +1,227 @@
+"""GPU parity: unary transform_reduce and transform_inclusive_scan /
+transform_exclusive_scan (numeric.hpp:193-212, 299-327) against the oracle.
+
+Integer results are bit-exact for every element type, operator and
+transform.  Floating-point inputs are small integers, so sums, minima and
+maxima of their transforms are exact in any order and compare bit-exactly;
+products use values near 1 and compare within 4 * n * eps relative.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import opensycl_amd as acpp
+import oracle as O
+from gpu_util import ALL_OPS, FP_OPS, FP_TYPES, INT_TYPES, dev, host, op_input, rand
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
+INT_CODES = list(range(8))
+FP_CODES = [0, 1, 2, 3, 4, 6, 7]
+SIZES = [1, 127, 1000, 8193, 100003]
+
+
+def _fp_input(dtype, op, n, seed):
+    rng = np.random.default_rng(seed)
+    if op == acpp.MULTIPLY:
+        return (1.0 + rng.standard_normal(n) * 1e-3).astype(dtype)
+    x = np.round(rng.standard_normal(n) * 8).astype(dtype)
+    x[rng.random(n) < 0.05] = 0
+    return x
+
+
+def _reduce(torch, q, scratch, x, init, op, code, config=None):
+    xd = dev(torch, x)
+    out = torch.full((1,), 77, dtype=xd.dtype, device="cuda")
+    acpp.transform_reduce_unary(q, scratch, xd, out, init, op, code, config=config)
+    q.wait()
+    return host(out)[0]
+
+
+def _scan(torch, q, scratch, x, op, code, kind, init=None):
+    xd = dev(torch, x)
+    yd = torch.empty_like(xd)
+    if kind == "inclusive":
+        acpp.transform_inclusive_scan(q, scratch, xd, yd, op, code, init)
+    else:
+        acpp.transform_exclusive_scan(q, scratch, xd, yd, init, op, code)
+    q.wait()
+    return host(yd)
+
+
+def _close(got, exp, n, dtype):
+    eps = float(np.finfo(dtype).eps)
+    got = np.asarray(got, np.float64)
+    exp = np.asarray(exp, np.float64)
+    same_inf = np.isinf(exp) & (got == exp)
+    with np.errstate(invalid="ignore"):  # inf - inf where both overflowed
+        ok = same_inf | (np.abs(got - exp) <= 4 * n * eps * np.abs(exp))
+    return bool(np.all(ok))
+
+
+@pytest.mark.parametrize("dtype", INT_TYPES)
+def test_transform_reduce_int_exact(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    for op in ALL_OPS:
+        for n in (1, 1000, 100003):
+            x = op_input(dtype, op, n, seed=n + op)
+            init = dtype(3)
+            for code in INT_CODES:
+                got = _reduce(torch, q, scratch, x, init, op, code)
+                assert got == O.transform_reduce(x, init, op, code), (dtype, op, n, code)
+
+
+@pytest.mark.parametrize("dtype", FP_TYPES)
+def test_transform_reduce_fp(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    for op in FP_OPS:
+        for n in (1, 1000, 100003):
+            x = _fp_input(dtype, op, n, seed=n + op)
+            init = dtype(3) if op != acpp.MULTIPLY else dtype(1)
+            for code in FP_CODES:
+                got = _reduce(torch, q, scratch, x, init, op, code)
+                exp = O.transform_reduce(x, init, op, code)
+                if op == acpp.MULTIPLY:
+                    assert _close(got, exp, n, dtype), (dtype, n, code, got, exp)
+                else:
+                    assert got == exp, (dtype, op, n, code, got, exp)
+
+
+@pytest.mark.parametrize("dtype", INT_TYPES)
+def test_transform_scans_int_exact(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    for op in ALL_OPS:
+        for n in SIZES:
+            x = op_input(dtype, op, n, seed=7 * n + op)
+            init = dtype(3)
+            for code in INT_CODES:
+                got = _scan(torch, q, scratch, x, op, code, "inclusive")
+                assert np.array_equal(got, O.transform_inclusive_scan(x, op, code)), (dtype, op, n, code)
+                got = _scan(torch, q, scratch, x, op, code, "inclusive", init)
+                assert np.array_equal(got, O.transform_inclusive_scan(x, op, code, init)), \
+                    (dtype, op, n, code, "init")
+                got = _scan(torch, q, scratch, x, op, code, "exclusive", init)
+                assert np.array_equal(got, O.transform_exclusive_scan(x, init, op, code)), \
+                    (dtype, op, n, code, "excl")
+
+
+@pytest.mark.parametrize("dtype", FP_TYPES)
+def test_transform_scans_fp(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    for op in FP_OPS:
+        for n in SIZES:
+            x = _fp_input(dtype, op, n, seed=3 * n + op)
+            init = dtype(3) if op != acpp.MULTIPLY else dtype(1)
+            for code in FP_CODES:
+                for kind in ("inclusive", "exclusive"):
+                    got = _scan(torch, q, scratch, x, op, code, kind, init)
+                    if kind == "inclusive":
+                        exp = O.transform_inclusive_scan(x, op, code, init)
+                    else:
+                        exp = O.transform_exclusive_scan(x, init, op, code)
+                    if op == acpp.MULTIPLY:
+                        assert _close(got, exp, n, dtype), (dtype, n, code, kind)
+                    else:
+                        assert np.array_equal(got, exp), (dtype, op, n, code, kind)
+
+
+def test_transform_misaligned_inputs(gpu, q, scratch):
+    # views that start off a 16-byte boundary take the scalar / partial paths
+    torch, _ = gpu
+    for dtype in (np.int8, np.int16, np.int32, np.int64):
+        x = rand(dtype, 100003, seed=11)
+        for off in (1, 3):
+            xd = dev(torch, x)[off:]
+            out = torch.zeros(1, dtype=xd.dtype, device="cuda")
+            acpp.transform_reduce_unary(q, scratch, xd, out, 0, acpp.PLUS, acpp.SQUARE)
+            yd = torch.empty_like(xd)
+            acpp.transform_inclusive_scan(q, scratch, xd, yd, acpp.PLUS, acpp.NEGATE)
+            q.wait()
+            assert host(out)[0] == O.transform_reduce(x[off:], dtype(0), O.PLUS, O.SQUARE)
+            assert np.array_equal(host(yd), O.transform_inclusive_scan(x[off:], O.PLUS, O.NEGATE))
+
+
+def test_golden_pstl_transform_scans(gpu, q, scratch):
+    # tests/pstl/transform_inclusive_scan.cpp / transform_exclusive_scan.cpp:
+    # data[i] = i, unary_op = op(x, x), plus with init 3 and multiplies with 3ull
+    torch, _ = gpu
+    for n in GOLDEN["pstl_transform_scans"]["sizes"]:
+        if n == 0:
+            continue
+        i = np.arange(n, dtype=np.int64)
+        w = lambda v: (v & 0xffffffff).astype(np.uint32).view(np.int32)  # noqa: E731
+        x = i.astype(np.int32)
+        assert np.array_equal(_scan(torch, q, scratch, x, acpp.PLUS, acpp.TWICE, "inclusive"), w(i * (i + 1)))
+        assert np.array_equal(_scan(torch, q, scratch, x, acpp.PLUS, acpp.TWICE, "inclusive", np.int32(3)),
+                              w(3 + i * (i + 1)))
+        assert np.array_equal(_scan(torch, q, scratch, x, acpp.PLUS, acpp.TWICE, "exclusive", np.int32(3)),
+                              w(3 + i * (i - 1)))
+        u = i.astype(np.uint64)
+        assert np.all(_scan(torch, q, scratch, u, acpp.MULTIPLY, acpp.SQUARE, "inclusive") == 0)
+        assert np.all(_scan(torch, q, scratch, u, acpp.MULTIPLY, acpp.SQUARE, "inclusive", np.uint64(3)) == 0)
+        ex = _scan(torch, q, scratch, u, acpp.MULTIPLY, acpp.SQUARE, "exclusive", np.uint64(3))
+        assert ex[0] == 3 and np.all(ex[1:] == 0)
+
+
+@pytest.mark.parametrize("case", GOLDEN["pstl_transform_reduce"]["cases"])
+def test_golden_pstl_transform_reduce(gpu, q, scratch, case):
+    torch, _ = gpu
+    dt = np.int64 if case.get("dtype") == "int64" else np.int32
+    n = case["n"]
+    xd = torch.arange(n, dtype=torch.int64 if dt == np.int64 else torch.int32, device="cuda")
+    out = torch.full((1,), case["init"], dtype=xd.dtype, device="cuda")
+    acpp.transform_reduce_unary(q, scratch, xd, out, dt(case["init"]), acpp.PLUS, acpp.IDENTITY)
+    q.wait()
+    assert host(out)[0] == case["expected"]
+
+
+def test_transform_large_properties(gpu, q, scratch):
+    # size-independent checks at 2^26 int32: the negated scan is the negation
+    # of the scan (linearity), and the sum of squares matches the oracle.
+    torch, _ = gpu
+    n = 1 << 26
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    xd = torch.randint(-(1 << 15), (1 << 15) + 1, (n,), dtype=torch.int32, device="cuda", generator=g)
+    yd = torch.empty_like(xd)
+    zd = torch.empty_like(xd)
+    acpp.transform_inclusive_scan(q, scratch, xd, yd, acpp.PLUS, acpp.NEGATE)
+    acpp.inclusive_scan(q, scratch, xd, zd)
+    out = torch.zeros(1, dtype=torch.int64, device="cuda")
+    x64 = xd.to(torch.int64)
+    acpp.transform_reduce_unary(q, scratch, x64, out, 0, acpp.PLUS, acpp.SQUARE)
+    q.wait()
+    assert torch.equal(yd, -zd)
+    assert host(out)[0] == O.transform_reduce(host(x64), np.int64(0), O.PLUS, O.SQUARE)
+
+
+def test_transform_errors_and_empty(gpu, q, scratch):
+    torch, _ = gpu
+    xf = torch.ones(16, dtype=torch.float32, device="cuda")
+    out = torch.full((1,), 5.0, dtype=torch.float32, device="cuda")
+    with pytest.raises(acpp.AcppError) as e:
+        acpp.transform_reduce_unary(q, scratch, xf, out, 0.0, acpp.PLUS, acpp.BIT_NOT)
+    assert e.value.status == 2
+    with pytest.raises(acpp.AcppError) as e:
+        acpp.transform_inclusive_scan(q, scratch, xf, xf, acpp.PLUS, 8)
+    assert e.value.status == 1
+    with pytest.raises(acpp.AcppError) as e:
+        acpp.transform_reduce_unary(q, scratch, xf, out, 0.0, acpp.PLUS, acpp.SQUARE,
+                                    config=(256, 8))
+    assert e.value.status == 1
+    # empty input leaves *out untouched (numeric.hpp:165-169)
+    acpp.transform_reduce_unary(q, scratch, xf[:0], out, 1.0, acpp.PLUS, acpp.SQUARE)
+    q.wait()
+    assert host(out)[0] == 5.0
+
+
+def test_transform_reduce_launch_configs(gpu, q, scratch):
+    torch, _ = gpu
+    x = rand(np.int32, 3 * 1024 * 1024 + 5, seed=2)
+    exp = O.transform_reduce(x, np.int32(0), O.PLUS, O.ABS)
+    for cfg in [(64, 4), (256, 4, 1), (1024, 4, 2), (512, 4, 0, 0)]:
+        assert _reduce(torch, q, scratch, x, np.int32(0), acpp.PLUS, acpp.ABS, config=cfg) == exp, cfg
