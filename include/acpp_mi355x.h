@@ -149,6 +149,13 @@ acpp_status acpp_allocation_group_obtain(acpp_allocation_group *group,
 acpp_status acpp_allocation_group_release(acpp_allocation_group *group);
 /* ~allocation_group -- allocation_cache.hpp:158-160 */
 acpp_status acpp_allocation_group_destroy(acpp_allocation_group *group);
+/* No counterpart in the reference (its look-back spins without bound).
+   Look-backs of the scans, copy_if and segmented scans give up after ~2.7 s
+   of waiting instead of hanging the device; such a give-up makes that
+   call's results wrong.  *trips = the number of give-ups recorded in the
+   scratch regions the group holds (since each region was last reset).
+   Blocking; call once the group's work has completed.  Expected to be 0. */
+acpp_status acpp_allocation_group_watchdog_trips(acpp_allocation_group *group, uint64_t *trips);
 
 /* Number of compute units of `device` (the value the reference reads as
    info::device::max_compute_units, numeric.hpp:154-156). */

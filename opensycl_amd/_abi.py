@@ -55,6 +55,7 @@ def _signatures() -> dict[str, tuple[list, object]]:
     S["acpp_allocation_group_obtain"] = ([vp, c_size_t, c_size_t, POINTER(vp)], c_int)
     S["acpp_allocation_group_release"] = ([vp], c_int)
     S["acpp_allocation_group_destroy"] = ([vp], c_int)
+    S["acpp_allocation_group_watchdog_trips"] = ([vp, POINTER(c_uint64)], c_int)
     for tag, ct in SCALAR_TAGS.items():
         S[f"acpp_reduce_{tag}"] = ([vp, vp, vp, c_uint64, vp, ct, c_int32, cfg], c_int)
         S[f"acpp_reduction_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_int32, c_int, cfg], c_int)

@@ -174,6 +174,14 @@ class AllocationGroup:
     def release(self) -> None:
         check(_abi.load().acpp_allocation_group_release(self._h))
 
+    def watchdog_trips(self) -> int:
+        """Look-backs that gave up waiting in the scratch regions this group
+        holds (blocking; call after the group's work has completed).  A
+        non-zero count means some results are wrong; expected to be 0."""
+        v = ctypes.c_uint64()
+        check(_abi.load().acpp_allocation_group_watchdog_trips(self._h, ctypes.byref(v)))
+        return v.value
+
     def close(self) -> None:
         if self._h:
             check(_abi.load().acpp_allocation_group_destroy(self._h))

@@ -53,15 +53,56 @@ struct sync_header {
   uint32_t pad[60];         // header occupies 256 bytes
 };
 
-// A look-back that polls this many times without progress gives up (and
-// counts a watchdog trip) instead of spinning forever.  Predecessor tiles
-// are always resident, so this only fires if the hardware stops making
-// progress; it keeps such a failure from hanging the device.
-constexpr uint32_t lookback_poll_limit = 1u << 22;
 static_assert(sizeof(sync_header) == 256, "sync header is 256 bytes");
 
-// Epochs live in 30 bits of a status word; 0 is never a live epoch.
-constexpr uint32_t max_epoch = (1u << 30) - 1;
+// Waiting inside a decoupled look-back.
+//   - Backoff: a poll round that finds an unfinished predecessor sleeps
+//     before polling again, twice as long each time up to 64 s_sleep(1)
+//     units (~4k clocks).  With thousands of small tiles in flight,
+//     back-to-back polls flood the memory system and starve the very tiles
+//     being waited for: 65,536-segment scans with 256-element tiles averaged
+//     over 2e5 polls per look-back on MI355X without it.
+//   - Watchdog: the wait is bounded in wall-clock time (the steady 100 MHz
+//     counter), not in polls, so a slow but live chain never trips it.
+//     After lookback_timeout_ticks the look-back gives up, counts a trip in
+//     sync_header::watchdog_trips and continues with what it has (the
+//     results are then wrong; acpp_allocation_group_watchdog_trips reports
+//     it).  It only guards against a device that stops making progress.
+constexpr uint64_t lookback_timeout_ticks = 1ull << 28;  // ~2.7 s at 100 MHz
+constexpr uint32_t lookback_max_backoff = 64;
+
+struct lookback_wait {
+  uint64_t start;
+  uint32_t backoff;
+  __device__ __forceinline__ void begin() {
+    start = static_cast<uint64_t>(wall_clock64());
+    backoff = 0;
+  }
+  // After an unsuccessful round: back off, then report whether the wait has
+  // run out of time.  The result is uniform across the wave.
+  __device__ __forceinline__ bool failed_round() {
+    backoff = backoff ? (backoff < lookback_max_backoff ? 2 * backoff : backoff) : 1;
+    for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(1);
+    return static_cast<uint64_t>(wall_clock64()) - start > lookback_timeout_ticks;
+  }
+};
+
+// Tag of a look-back status word (the upper 32 bits of every 64-bit status
+// word): epoch << 3 | head flag << 2 | status (2 bits).  All look-back
+// kernels (scans, copy_if, segmented scans) use this one layout because they
+// share cached scratch regions and each region's epoch counter: a word left
+// by an earlier launch of any of them carries an older epoch and never
+// reads as valid.  (With per-kernel layouts, a segmented-scan word of epoch
+// e decoded as a scan word of epoch 2e + head, and was taken as live.)
+__host__ __device__ constexpr uint32_t status_tag(uint32_t epoch, uint32_t head, uint32_t status) {
+  return (epoch << 3) | (head << 2) | status;
+}
+__host__ __device__ constexpr uint32_t tag_epoch(uint32_t tag) { return tag >> 3; }
+__host__ __device__ constexpr uint32_t tag_head(uint32_t tag) { return (tag >> 2) & 1u; }
+__host__ __device__ constexpr uint32_t tag_status(uint32_t tag) { return tag & 3u; }
+
+// Epochs live in 29 bits of a tag; 0 is never a live epoch.
+constexpr uint32_t max_epoch = (1u << 29) - 1;
 
 }  // namespace acpp_impl
 

@@ -8,6 +8,7 @@
 // pool of "sync regions" (zeroed once, kept reusable by the kernels), which
 // is what lets the scan and the reduction run as single launches.
 #include <algorithm>
+#include <cstddef>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -284,6 +285,20 @@ acpp_status acpp_allocation_group_destroy(acpp_allocation_group *group) {
   if (!group) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "group is NULL");
   acpp_allocation_group_release(group);
   delete group;
+  return clear_error();
+}
+
+acpp_status acpp_allocation_group_watchdog_trips(acpp_allocation_group *group, uint64_t *trips) {
+  if (!group || !trips) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "group or trips is NULL");
+  uint64_t total = 0;
+  for (const allocation &a : group->held) {
+    if (!a.sync) continue;
+    uint32_t v = 0;
+    const char *src = static_cast<const char *>(a.ptr) + offsetof(sync_header, watchdog_trips);
+    ACPP_CHECK_HIP(hipMemcpy(&v, src, sizeof(v), hipMemcpyDeviceToHost));
+    total += v;
+  }
+  *trips = total;
   return clear_error();
 }
 

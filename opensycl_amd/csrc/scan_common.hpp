@@ -20,7 +20,7 @@
 //     per-lane totals with DPP, and one LDS exchange of the wave totals
 //     (one barrier) gives every work-item its offset inside the tile;
 //   - the status of a tile is self-validating: value bits and a tag
-//     (epoch << 2 | flag) share one 64-bit word (two words for 64-bit
+//     (status_tag in internal.hpp) share one 64-bit word (two words for 64-bit
 //     values, each carrying the tag), written and read with relaxed
 //     device-scope atomics.  Words from earlier launches carry an older
 //     epoch and read as "invalid", so nothing has to be cleared;
@@ -37,7 +37,9 @@
 //     region, the epoch of the last launch and how many tile tickets have
 //     been drawn, and passes both as kernel arguments (a per-tile "done"
 //     counter would add a second same-address atomic per tile);
-//   - a bounded-spin watchdog keeps a look-back from spinning forever.
+//   - a poll round that finds an unfinished predecessor backs off before
+//     the next, and a wall-clock watchdog keeps a look-back from spinning
+//     forever (lookback_wait in internal.hpp).
 #pragma once
 
 #include "internal.hpp"
@@ -71,7 +73,7 @@ struct status_io<W, 4> {
   static constexpr int words = 1;
   __device__ __forceinline__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch,
                                                  uint32_t flag, W v) {
-    const uint64_t tag = (static_cast<uint64_t>(epoch) << 2) | flag;
+    const uint64_t tag = status_tag(epoch, 0, flag);
     atomic_store_u64(st + static_cast<uint64_t>(tile) * status_stride_words,
                      (tag << 32) | __builtin_bit_cast(uint32_t, v));
   }
@@ -79,9 +81,9 @@ struct status_io<W, 4> {
                                                   uint32_t epoch, W &v) {
     const uint64_t w = atomic_load_u64(st + static_cast<uint64_t>(tile) * status_stride_words);
     const uint32_t tag = static_cast<uint32_t>(w >> 32);
-    if ((tag >> 2) != epoch) return st_invalid;
+    if (tag_epoch(tag) != epoch) return st_invalid;
     v = __builtin_bit_cast(W, static_cast<uint32_t>(w));
-    return tag & 3u;
+    return tag_status(tag);
   }
 };
 
@@ -90,7 +92,7 @@ struct status_io<W, 8> {
   static constexpr int words = 2;
   __device__ __forceinline__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch,
                                                  uint32_t flag, W v) {
-    const uint64_t tag = (static_cast<uint64_t>(epoch) << 2) | flag;
+    const uint64_t tag = status_tag(epoch, 0, flag);
     const uint64_t bits = __builtin_bit_cast(uint64_t, v);
     uint64_t *slot = st + static_cast<uint64_t>(tile) * status_stride_words;
     atomic_store_u64(slot, (tag << 32) | (bits & 0xffffffffu));
@@ -104,9 +106,9 @@ struct status_io<W, 8> {
     const uint32_t t0 = static_cast<uint32_t>(w0 >> 32);
     const uint32_t t1 = static_cast<uint32_t>(w1 >> 32);
     // Both halves must come from the same publication.
-    if (t0 != t1 || (t0 >> 2) != epoch) return st_invalid;
+    if (t0 != t1 || tag_epoch(t0) != epoch) return st_invalid;
     v = __builtin_bit_cast(W, (w1 << 32) | (w0 & 0xffffffffu));
-    return t0 & 3u;
+    return tag_status(t0);
   }
 };
 
@@ -119,7 +121,9 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
   const int lane = wave::lane_id();
   W excl = Op::identity();
   int64_t base = static_cast<int64_t>(tile) - 1;
-  uint32_t polls = 0;
+  lookback_wait wait;
+  wait.begin();
+  bool give_up = false;  // uniform
   for (;;) {
     const int64_t j = base - lane;  // lane 0 = nearest predecessor
     W val = Op::identity();
@@ -128,16 +132,16 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
       uint32_t flag = st_prefix;  // positions before tile 0 act as an empty prefix
       if (j >= 0) flag = SIO::read(status, static_cast<uint32_t>(j), epoch, val);
       if (j < 0) val = Op::identity();
-      if (++polls > lookback_poll_limit) {
-        if (lane == 0 && polls == lookback_poll_limit + 1)
-          __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        flag = st_prefix;
-      }
+      if (give_up) flag = st_prefix;
       const uint64_t pm = wave::ballot(flag == st_prefix);
       p = pm ? __builtin_ctzll(pm) : 64;
       const uint64_t need = (p >= 63) ? ~0ull : ((2ull << p) - 1);
       const uint64_t bad = wave::ballot(flag == st_invalid) & need;
       if (bad == 0) break;
+      if (wait.failed_round()) {
+        give_up = true;
+        if (lane == 0) __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     const W x = (lane <= p) ? val : Op::identity();
     const W window = wave::reduce_full<Op>(x);
@@ -165,7 +169,10 @@ __device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoc
   const int lane = tid & 63;
   W excl = Op::identity();
   int64_t base = static_cast<int64_t>(tile) - 1;
-  uint32_t polls = 0;
+  __shared__ int s_give_up;
+  lookback_wait wait;
+  wait.begin();
+  bool give_up = false;  // uniform across the work-group
   for (;;) {
     const int64_t j = base - tid;  // work-item 0 = nearest predecessor
     W val = Op::identity();
@@ -174,11 +181,7 @@ __device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoc
       uint32_t flag = st_prefix;  // positions before tile 0 act as an empty prefix
       if (j >= 0) flag = SIO::read(status, static_cast<uint32_t>(j), epoch, val);
       if (j < 0) val = Op::identity();
-      if (++polls > lookback_poll_limit) {
-        if (tid == 0 && polls == lookback_poll_limit + 1)
-          __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        flag = st_prefix;
-      }
+      if (give_up) flag = st_prefix;
       const uint64_t pm = wave::ballot(flag == st_prefix);
       const uint64_t im = wave::ballot(flag == st_invalid);
       if (lane == 0) {
@@ -205,7 +208,14 @@ __device__ W block_lookback(const uint64_t *status, uint32_t tile, uint32_t epoc
       }
       __syncthreads();  // the masks are rewritten by the next poll
       if (!bad) break;
-      __builtin_amdgcn_s_sleep(1);
+      // Every work-item backs off; work-item 0's clock decides for all.
+      const bool late = wait.failed_round();
+      if (tid == 0) {
+        s_give_up = late ? 1 : 0;
+        if (late) __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+      give_up = s_give_up != 0;
     }
     // Fold the window [nearest .. p] and broadcast it.
     W x = (tid <= p) ? val : Op::identity();

@@ -23,8 +23,10 @@
 //     and a look-back stops at the nearest predecessor that contains a
 //     segment start (its aggregate already discards everything before it),
 //     so it is short whenever segments are shorter than a tile;
-//   - reductions write op(init, value) at the last element of each segment
-//     only; scans write every element;
+//   - scans write every element.  Reductions skip the look-back: each tile
+//     writes the totals of the segments it holds whole and two partial
+//     folds, and a fix-up kernel joins the partials of segments that span
+//     tiles (segmented_reduce_kernel below);
 //   - the setup and main kernels can be captured into a hipGraph once and
 //     replayed (acpp_segmented_plan_*).
 #include <memory>
@@ -39,8 +41,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 enum : uint32_t { st_invalid = 0, st_aggregate = 1, st_prefix = 2 };
 
-// Tag layout: epoch (29 bits) << 3 | head flag << 2 | status (2 bits).
-constexpr uint32_t seg_max_epoch = (1u << 29) - 1;
+// Status tags: status_tag() in internal.hpp (epoch << 3 | head flag << 2 |
+// status), shared with the scans.
 
 __device__ __forceinline__ uint64_t ld64(const uint64_t *p) {
   return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -61,17 +63,17 @@ struct seg_status<W, 4> {
   static constexpr int words = 1;
   __device__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t status,
                                  uint32_t f, W v) {
-    const uint64_t tag = (static_cast<uint64_t>(epoch) << 3) | (f << 2) | status;
+    const uint64_t tag = status_tag(epoch, f, status);
     st64(st + tile * seg_stride_words, (tag << 32) | __builtin_bit_cast(uint32_t, v));
   }
   __device__ static uint32_t read(const uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t &f,
                                   W &v) {
     const uint64_t w = ld64(st + tile * seg_stride_words);
     const uint32_t tag = static_cast<uint32_t>(w >> 32);
-    if ((tag >> 3) != epoch) return st_invalid;
-    f = (tag >> 2) & 1u;
+    if (tag_epoch(tag) != epoch) return st_invalid;
+    f = tag_head(tag);
     v = __builtin_bit_cast(W, static_cast<uint32_t>(w));
-    return tag & 3u;
+    return tag_status(tag);
   }
 };
 
@@ -80,7 +82,7 @@ struct seg_status<W, 8> {
   static constexpr int words = 2;
   __device__ static void publish(uint64_t *st, uint32_t tile, uint32_t epoch, uint32_t status,
                                  uint32_t f, W v) {
-    const uint64_t tag = (static_cast<uint64_t>(epoch) << 3) | (f << 2) | status;
+    const uint64_t tag = status_tag(epoch, f, status);
     const uint64_t bits = __builtin_bit_cast(uint64_t, v);
     uint64_t *slot = st + tile * seg_stride_words;
     st64(slot, (tag << 32) | (bits & 0xffffffffu));
@@ -93,10 +95,10 @@ struct seg_status<W, 8> {
     const uint64_t w1 = ld64(slot + 1);
     const uint32_t t0 = static_cast<uint32_t>(w0 >> 32);
     const uint32_t t1 = static_cast<uint32_t>(w1 >> 32);
-    if (t0 != t1 || (t0 >> 3) != epoch) return st_invalid;
-    f = (t0 >> 2) & 1u;
+    if (t0 != t1 || tag_epoch(t0) != epoch) return st_invalid;
+    f = tag_head(t0);
     v = __builtin_bit_cast(W, (w1 << 32) | (w0 & 0xffffffffu));
-    return t0 & 3u;
+    return tag_status(t0);
   }
 };
 
@@ -126,6 +128,49 @@ __device__ __forceinline__ void wave_pair_scan(uint32_t &f, W &v) {
 #undef ACPP_SEG_STEP
 }
 
+// Exclusive prefixes, in element order, of a tile's per-(row, wave) pair
+// totals.  s_tf / s_tv[r * nw + w] hold the totals on entry and the
+// exclusive prefixes on return (P = R * nw <= 128 entries).  Called by one
+// whole wave, which scans them with DPP (two entries per lane above 64);
+// returns the tile aggregate, uniform across the wave.  Replaces a serial
+// walk over all R * nw totals by every work-item.
+template <class Op, class W>
+__device__ void tile_pair_prefixes(int P, uint32_t *s_tf, W *s_tv, uint32_t &agg_f, W &agg_v) {
+  const int lane = threadIdx.x & 63;
+  const W id = Op::identity();
+  const int per = P > 64 ? 2 : 1;
+  const int a = lane * per;
+  uint32_t f0 = 0, f1 = 0;
+  W v0 = id, v1 = id;
+  if (a < P) {
+    f0 = s_tf[a];
+    v0 = s_tv[a];
+  }
+  if (per == 2 && a + 1 < P) {
+    f1 = s_tf[a + 1];
+    v1 = s_tv[a + 1];
+  }
+  uint32_t sf = f1;  // lane total: entry a . entry a + 1
+  W sv = v1;
+  combine<Op>(f0, v0, sf, sv);
+  wave_pair_scan<Op>(sf, sv);
+  const uint32_t xf = wave::dpp<wave::wave_shr1, 0xf, 0xf>(0u, sf);
+  const W xv = wave::dpp<wave::wave_shr1, 0xf, 0xf>(id, sv);
+  agg_f = wave::read_lane(sf, 63);
+  agg_v = wave::read_lane(sv, 63);
+  if (a < P) {
+    s_tf[a] = xf;
+    s_tv[a] = xv;
+  }
+  if (per == 2 && a + 1 < P) {
+    uint32_t bf = f0;
+    W bv = v0;
+    combine<Op>(xf, xv, bf, bv);
+    s_tf[a + 1] = bf;
+    s_tv[a + 1] = bv;
+  }
+}
+
 // Ordered look-back by one wave: lane l polls tile (wbase - 63 + l), lane 63
 // the nearest predecessor.  It stops at the nearest tile that is a prefix
 // or an aggregate containing a segment start, and folds the window from
@@ -139,7 +184,9 @@ __device__ void wave_pair_lookback(const uint64_t *status, uint32_t tile, uint32
   int64_t wbase = static_cast<int64_t>(tile) - 1;
   uint32_t ef = 0;  // accumulated prefix of tiles nearer than the window
   W ev = id;
-  uint32_t polls = 0;
+  lookback_wait wait;
+  wait.begin();
+  bool give_up = false;  // uniform
   for (;;) {
     const int64_t j = wbase - (63 - lane);
     uint32_t lf = 0;
@@ -150,17 +197,17 @@ __device__ void wave_pair_lookback(const uint64_t *status, uint32_t tile, uint32
       lf = 0;
       lv = id;
       if (j >= 0) st = SIO::read(status, static_cast<uint32_t>(j), epoch, lf, lv);
-      if (++polls > lookback_poll_limit) {
-        if (lane == 0 && polls == lookback_poll_limit + 1)
-          __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        st = st_prefix;
-      }
+      if (give_up) st = st_prefix;
       const bool term = (st == st_prefix) || (st == st_aggregate && lf);
       const uint64_t tm = wave::ballot(term);
       q = tm ? 63 - __builtin_clzll(tm) : -1;
       const uint64_t need = (q <= 0) ? ~0ull : (~0ull << q);  // lanes q..63
       const uint64_t bad = wave::ballot(st == st_invalid) & need;
       if (bad == 0) break;
+      if (wait.failed_round()) {
+        give_up = true;
+        if (lane == 0) __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     uint32_t xf = (lane >= q) ? lf : 0u;
     W xv = (lane >= q) ? lv : id;
@@ -255,11 +302,14 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
 
   extern __shared__ uint32_t s_heads[];  // tile_elems + 1 bits
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_wf[R][max_waves];
-  __shared__ W s_wv[R][max_waves];
+  __shared__ uint32_t s_tf[R * max_waves];  // per (row, wave): totals, then exclusive prefixes
+  __shared__ W s_tv[R * max_waves];
+  __shared__ uint32_t s_aggf;
+  __shared__ W s_aggv;
   __shared__ uint32_t s_lbf[max_waves];
   __shared__ W s_lbv[max_waves];
   __shared__ int s_q[max_waves];
+  __shared__ int s_give_up;
   __shared__ uint64_t s_invalid[max_waves];
   const uint32_t head_words = static_cast<uint32_t>((tile_elems + 1 + 31) / 32);
 
@@ -290,11 +340,6 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
       if (static_cast<uint32_t>(tid + B) < head_words && i1 < p.head_words_total) hword1 = p.heads[i1];
     }
     __builtin_amdgcn_sched_barrier(0);
-    // Segments touching the tile: [s_lo, s_hi] (used to number the outputs
-    // of a reduction).  Indices are clamped so that malformed offsets can
-    // give wrong results but never out-of-bounds accesses.
-    uint64_t s_lo = p.tile_seg[tile];
-    uint64_t s_hi = p.tile_seg[tile + 1];
 
     // Load the tile.
     const bool full = p.vec_io && (base + tile_elems <= p.total);
@@ -315,9 +360,6 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
     }
     if (static_cast<uint32_t>(tid) < head_words) s_heads[tid] = hword0;
     if (static_cast<uint32_t>(tid + B) < head_words) s_heads[tid + B] = hword1;
-    if (s_lo >= p.num_segments) s_lo = p.num_segments - 1;
-    if (s_hi >= p.num_segments) s_hi = p.num_segments - 1;
-    if (s_hi < s_lo) s_hi = s_lo;
     __syncthreads();
     if (full) {
 #pragma unroll
@@ -360,46 +402,47 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
       W tv = v[r][VEC - 1];
       wave_pair_scan<Op>(tf, tv);
       if (lane == 63) {
-        s_wf[r][w] = tf;
-        s_wv[r][w] = tv;
+        s_tf[r * nw + w] = tf;
+        s_tv[r * nw + w] = tv;
       }
       xf[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(0u, tf);
       xv[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(id, tv);
     }
     __syncthreads();
+    // Wave 0 turns the (row, wave) totals into exclusive prefixes and
+    // publishes the tile's aggregate at once.  A tile that contains a
+    // segment start publishes it as its prefix: it is final for everything
+    // after the tile.
+    if (w == 0) {
+      uint32_t af;
+      W av;
+      tile_pair_prefixes<Op, W>(R * nw, s_tf, s_tv, af, av);
+      if (lane == 0) {
+        s_aggf = af;
+        s_aggv = av;
+        if (tile == 0) {
+          if (p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, af, av);
+        } else {
+          SIO::publish(p.status, tile, p.epoch, af ? st_prefix : st_aggregate, af, av);
+        }
+      }
+    }
+    __syncthreads();
     // Pairs before (row r, wave w) inside the tile, in element order.
     uint32_t bf[R];
     W bv[R];
-    uint32_t cf = 0;
-    W cv = id;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      bf[r] = 0;
-      bv[r] = id;
-      for (int ww = 0; ww < nw; ++ww) {
-        if (ww == w) {
-          bf[r] = cf;
-          bv[r] = cv;
-        }
-        uint32_t yf = s_wf[r][ww];
-        W yv = s_wv[r][ww];
-        combine<Op>(cf, cv, yf, yv);
-        cf = yf;
-        cv = yv;
-      }
+      bf[r] = s_tf[r * nw + w];
+      bv[r] = s_tv[r * nw + w];
     }
-    const uint32_t agg_f = cf;
-    const W agg_v = cv;
+    const uint32_t agg_f = s_aggf;
+    const W agg_v = s_aggv;
 
     // Tile prefix: ordered look-back (one wave, or the whole work-group).
     uint32_t pf = 0;
     W pv = id;
-    if (tile == 0) {
-      if (tid == 0 && p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, agg_f, agg_v);
-    } else {
-      // A tile that contains a segment start publishes its prefix right away:
-      // its aggregate is final for everything after it.
-      if (tid == 0) SIO::publish(p.status, tile, p.epoch, agg_f ? st_prefix : st_aggregate, agg_f, agg_v);
+    if (tile != 0) {
       // Only elements before the tile's first segment start need the
       // prefix; a tile that begins with a segment start needs none.
       if (!head_at(0) && !p.block_lookback) {
@@ -427,7 +470,9 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
         int64_t wbase = static_cast<int64_t>(tile) - 1;
         uint32_t ef = 0;  // accumulated prefix of tiles nearer than the window
         W ev = id;
-        uint32_t polls = 0;
+        lookback_wait wait;
+        wait.begin();
+        bool give_up = false;  // uniform across the work-group
         for (;;) {
           const int64_t j = wbase - (B - 1 - tid);
           uint32_t lf = 0;
@@ -438,12 +483,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
             lf = 0;
             lv = id;
             if (j >= 0) st = SIO::read(p.status, static_cast<uint32_t>(j), p.epoch, lf, lv);
-            if (++polls > lookback_poll_limit) {
-              if (tid == 0 && polls == lookback_poll_limit + 1)
-                __hip_atomic_fetch_add(&p.sync->watchdog_trips, 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-              st = st_prefix;
-            }
+            if (give_up) st = st_prefix;
             const bool term = (st == st_prefix) || (st == st_aggregate && lf);
             const uint64_t tm = wave::ballot(term);
             const uint64_t im = wave::ballot(st == st_invalid);
@@ -469,7 +509,15 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
             }
             __syncthreads();
             if (!bad) break;
-            __builtin_amdgcn_s_sleep(1);
+            // Every work-item backs off; work-item 0's clock decides for all.
+            const bool late = wait.failed_round();
+            if (tid == 0) {
+              s_give_up = late ? 1 : 0;
+              if (late)
+                __hip_atomic_fetch_add(&p.sync->watchdog_trips, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            give_up = s_give_up != 0;
           }
           // Fold window tiles [q .. B-1] in tile order.
           uint32_t xf2 = (tid >= q) ? lf : 0;
@@ -535,32 +583,228 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
       }
       (void)Pf;
       const uint64_t e0 = base + static_cast<uint64_t>(r * B + tid) * VEC;
-      if (!p.reduce) {
-        if (full) {
-          T ot[VEC];
+      if (full) {
+        T ot[VEC];
 #pragma unroll
-          for (int k = 0; k < VEC; ++k) ot[k] = static_cast<T>(o[k]);
-          u32x4 qv;
-          __builtin_memcpy(&qv, ot, 16);
-          reinterpret_cast<u32x4 *>(p.out + base)[r * B + tid] = qv;
-        } else {
-#pragma unroll
-          for (int k = 0; k < VEC; ++k)
-            if (e0 + k < p.total) p.out[e0 + k] = static_cast<T>(o[k]);
-        }
+        for (int k = 0; k < VEC; ++k) ot[k] = static_cast<T>(o[k]);
+        u32x4 qv;
+        __builtin_memcpy(&qv, ot, 16);
+        reinterpret_cast<u32x4 *>(p.out + base)[r * B + tid] = qv;
       } else {
 #pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-          const uint64_t e = e0 + k;
-          if (e >= p.total) continue;
-          const bool last = (e + 1 == p.total) || head_at(static_cast<uint64_t>(r * B + tid) * VEC + k + 1);
-          if (last) {
-            const uint64_t s = find_segment(p.offsets, s_lo, s_hi, e, p.total);
-            p.out[s] = static_cast<T>(Op::apply(p.init, o[k]));
-          }
-        }
+        for (int k = 0; k < VEC; ++k)
+          if (e0 + k < p.total) p.out[e0 + k] = static_cast<T>(o[k]);
       }
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Segmented reduction without a look-back.
+//
+// A reduction needs only each segment's total, so tiles do not wait for
+// each other.  Every tile (one work-group, launched in any order) runs the
+// same in-tile (head flag, value) scan as the scan kernel and then
+//   - writes op(init, total) for every segment that starts and ends inside
+//     the tile;
+//   - stores head_part[tile]: the fold of the elements before its first
+//     segment start (the whole tile if it has none);
+//   - stores tail_part[tile]: the fold from its last segment start to the
+//     tile's end, when that segment continues into the next tile.
+// A fix-up kernel then gives every segment that spans tiles tb < ... <= tl
+// the total tail_part[tb] . head_part[tb+1] . ... . head_part[tl], folded in
+// tile order (deterministic).  The tiles strictly inside such a segment hold
+// no segment start, so their head_part is the whole tile.
+// --------------------------------------------------------------------------
+template <class T, class Op, int R>
+__global__ __launch_bounds__(1024) void segmented_reduce_kernel(params<T> p,
+                                                                typename elem_traits<T>::work *head_part,
+                                                                typename elem_traits<T>::work *tail_part) {
+  using W = typename elem_traits<T>::work;
+  constexpr int VEC = 16 / sizeof(T);
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int nw = B >> 6;
+  const int w = tid >> 6;
+  const int lane = tid & 63;
+  const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
+  const W id = Op::identity();
+
+  extern __shared__ uint32_t s_heads[];  // tile_elems + 1 bits
+  __shared__ uint32_t s_tf[R * max_waves];  // per (row, wave): totals, then exclusive prefixes
+  __shared__ W s_tv[R * max_waves];
+  // offsets[s_lo .. s_lo + off_cap) of the segments touching the tile: the
+  // binary search that numbers a finished segment then runs in LDS instead
+  // of as a chain of dependent global loads.
+  constexpr uint32_t off_cap = 1024;
+  __shared__ uint64_t s_off[off_cap];
+  const uint32_t head_words = static_cast<uint32_t>((tile_elems + 1 + 31) / 32);
+
+  const uint32_t tile = blockIdx.x;
+  if (tile >= p.num_tiles) return;
+  const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+  uint32_t hword0 = 0, hword1 = 0;
+  {
+    const uint64_t i0 = (base >> 5) + tid;
+    const uint64_t i1 = i0 + B;
+    if (static_cast<uint32_t>(tid) < head_words && i0 < p.head_words_total) hword0 = p.heads[i0];
+    if (static_cast<uint32_t>(tid + B) < head_words && i1 < p.head_words_total) hword1 = p.heads[i1];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  uint64_t s_lo = p.tile_seg[tile];
+  uint64_t s_hi = p.tile_seg[tile + 1];
+
+  const bool full = p.vec_io && (base + tile_elems <= p.total);
+  W v[R][VEC];
+  u32x4 q[R];
+  if (full) {
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
+        v[r][k] = (idx < p.total) ? elem_traits<T>::to_work(p.in[idx]) : id;
+      }
+  }
+  if (static_cast<uint32_t>(tid) < head_words) s_heads[tid] = hword0;
+  if (static_cast<uint32_t>(tid + B) < head_words) s_heads[tid + B] = hword1;
+  if (s_lo >= p.num_segments) s_lo = p.num_segments - 1;
+  if (s_hi >= p.num_segments) s_hi = p.num_segments - 1;
+  if (s_hi < s_lo) s_hi = s_lo;
+  const bool off_in_lds = (s_hi - s_lo) < off_cap;
+  {
+    const uint64_t n_off = off_in_lds ? (s_hi - s_lo + 1) : 0;
+    for (uint64_t i = tid; i < n_off; i += B) s_off[i] = clamp_off(p.offsets, s_lo + i, p.total);
+  }
+  __syncthreads();
+  if (full) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      T tmp[VEC];
+      __builtin_memcpy(tmp, &q[r], 16);
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+    }
+  }
+
+  auto head_at = [&](uint64_t pos) -> uint32_t {
+    return (s_heads[pos >> 5] >> (pos & 31)) & 1u;
+  };
+  // Largest s in [s_lo, s_hi] with offsets[s] <= e.
+  auto segment_of = [&](uint64_t e) -> uint64_t {
+    if (!off_in_lds) return find_segment(p.offsets, s_lo, s_hi, e, p.total);
+    uint64_t lo = 0, hi = s_hi - s_lo;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) / 2;
+      if (s_off[mid] <= e)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    return s_lo + lo;
+  };
+
+  uint32_t fm[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t pos = static_cast<uint64_t>(r * B + tid) * VEC;
+    fm[r] = (s_heads[pos >> 5] >> (pos & 31)) & ((1u << VEC) - 1u);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 1; k < VEC; ++k)
+      v[r][k] = ((fm[r] >> k) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
+
+  uint32_t xf[R];
+  W xv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    uint32_t tf = fm[r] != 0u;
+    W tv = v[r][VEC - 1];
+    wave_pair_scan<Op>(tf, tv);
+    if (lane == 63) {
+      s_tf[r * nw + w] = tf;
+      s_tv[r * nw + w] = tv;
+    }
+    xf[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(0u, tf);
+    xv[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(id, tv);
+  }
+  __syncthreads();
+  if (w == 0) {
+    uint32_t af;
+    W av;
+    tile_pair_prefixes<Op, W>(R * nw, s_tf, s_tv, af, av);
+    (void)af;
+    (void)av;
+  }
+  __syncthreads();
+  const uint64_t tile_left = p.total - base;  // > 0: the tile holds elements
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    // (row r, wave w) prefix inside the tile . wave-exclusive prefix
+    uint32_t Pf = s_tf[r * nw + w];
+    W Pv = s_tv[r * nw + w];
+    {
+      uint32_t tf = xf[r];
+      W tv = xv[r];
+      combine<Op>(Pf, Pv, tf, tv);
+      Pf = tf;
+      Pv = tv;
+    }
+    const uint64_t l0 = static_cast<uint64_t>(r * B + tid) * VEC;  // index inside the tile
+    // Bit k: element k of the vector is the last of its segment in this
+    // tile (a segment start follows, or the data or the tile ends there).
+    uint32_t ends = (fm[r] >> 1) | (head_at(l0 + VEC) << (VEC - 1));
+    if (l0 + VEC == tile_elems) ends |= 1u << (VEC - 1);
+    if (l0 < tile_left && tile_left <= l0 + VEC) ends |= 1u << (tile_left - 1 - l0);
+    if (tile_left <= l0) ends = 0;
+    else if (tile_left < l0 + VEC) ends &= (1u << (tile_left - l0)) - 1u;
+    if (ends == 0) continue;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      if (!((ends >> k) & 1u)) continue;
+      const uint64_t e = base + l0 + k;
+      // a segment start among elements 0..k of the vector cuts the prefix off
+      const bool cut = (fm[r] & ((2u << k) - 1u)) != 0u;
+      const W val = cut ? v[r][k] : Op::apply(Pv, v[r][k]);
+      const bool next_head = (e + 1 == p.total) || head_at(l0 + k + 1);
+      if (cut || Pf) {  // a segment start lies in [tile start, e]
+        if (next_head) {
+          const uint64_t s = segment_of(e);
+          p.out[s] = static_cast<T>(Op::apply(p.init, val));
+        } else {
+          tail_part[tile] = val;  // the tile's last element; the segment goes on
+        }
+      } else {
+        head_part[tile] = val;
+      }
+    }
+  }
+}
+
+// op(init, tail_part[tb] . head_part[tb+1] . ... . head_part[tl]) for every
+// segment whose elements span tiles tb < tl.
+template <class T, class Op>
+__global__ void segmented_fixup_kernel(params<T> p, const typename elem_traits<T>::work *head_part,
+                                       const typename elem_traits<T>::work *tail_part, uint64_t tile_elems) {
+  using W = typename elem_traits<T>::work;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < p.num_segments;
+       s += stride) {
+    const uint64_t b = clamp_off(p.offsets, s, p.total);
+    const uint64_t e = clamp_off(p.offsets, s + 1, p.total);
+    if (e <= b) continue;
+    const uint64_t tb = b / tile_elems;
+    const uint64_t tl = (e - 1) / tile_elems;
+    if (tb == tl || tl >= p.num_tiles) continue;
+    W acc = tail_part[tb];
+    for (uint64_t t = tb + 1; t <= tl; ++t) acc = Op::apply(acc, head_part[t]);
+    p.out[s] = static_cast<T>(Op::apply(p.init, acc));
   }
 }
 
@@ -606,6 +850,64 @@ acpp_status launch_rows(launch_plan<T, Op> &lp, hipStream_t stream, const char *
   return check_launch(name);
 }
 
+// Segmented reduction: clear the head bitmap, set up, one work-group per
+// tile, then the fix-up of segments that span tiles.
+template <class T, class Op>
+acpp_status enqueue_reduce(acpp_allocation_group *g, launch_plan<T, Op> &lp, hipStream_t stream, const T *in,
+                           T *out, const uint64_t *offsets, uint64_t num_segments, uint64_t total,
+                           typename elem_traits<T>::work init, uint64_t num_tiles, const char *name) {
+  using W = typename elem_traits<T>::work;
+  void *parts = nullptr;
+  acpp_status st = group_obtain(g, 2 * num_tiles * sizeof(W), &parts);
+  if (st != ACPP_SUCCESS) return st;
+  W *head_part = static_cast<W *>(parts);
+  W *tail_part = head_part + num_tiles;
+  params<T> &p = lp.p;
+  p.in = in;
+  p.out = out;
+  p.offsets = offsets;
+  p.tile_seg = lp.tile_seg;
+  p.heads = lp.heads;
+  p.head_words_total = lp.head_words;
+  p.block_lookback = 0;
+  p.num_segments = num_segments;
+  p.total = total;
+  p.init = init;
+  p.reduce = 1;
+  p.vec_io = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  p.num_tiles = static_cast<uint32_t>(num_tiles);
+  p.epoch = 0;
+  p.ticket_base = 0;
+  p.sync = nullptr;
+  p.status = nullptr;
+  lp.grid = static_cast<uint32_t>(num_tiles);
+
+  st = fill_zero(lp.heads, lp.head_words * sizeof(uint32_t), stream);
+  if (st != ACPP_SUCCESS) return st;
+  hipLaunchKernelGGL(setup_kernel<T>, dim3(static_cast<uint32_t>(std::min<uint64_t>(ceil_div(num_segments + 1, 256), 4096))),
+                     dim3(256), 0, stream, p, lp.tile_seg, lp.heads, lp.tile_elems);
+  st = check_launch(name);
+  if (st != ACPP_SUCCESS) return st;
+  if (total > 0) {
+    if (lp.rows == 2)
+      hipLaunchKernelGGL((segmented_reduce_kernel<T, Op, 2>), dim3(lp.grid), dim3(lp.wg), lp.lds, stream, p,
+                         head_part, tail_part);
+    else if (lp.rows == 4)
+      hipLaunchKernelGGL((segmented_reduce_kernel<T, Op, 4>), dim3(lp.grid), dim3(lp.wg), lp.lds, stream, p,
+                         head_part, tail_part);
+    else
+      hipLaunchKernelGGL((segmented_reduce_kernel<T, Op, 8>), dim3(lp.grid), dim3(lp.wg), lp.lds, stream, p,
+                         head_part, tail_part);
+    st = check_launch(name);
+    if (st != ACPP_SUCCESS) return st;
+    hipLaunchKernelGGL((segmented_fixup_kernel<T, Op>),
+                       dim3(static_cast<uint32_t>(std::min<uint64_t>(ceil_div(num_segments, 256), 4096))), dim3(256), 0,
+                       stream, p, head_part, tail_part, lp.tile_elems);
+    st = check_launch(name);
+  }
+  return st;
+}
+
 template <class T, class Op>
 acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T *out,
                     const uint64_t *offsets, uint64_t num_segments, uint64_t total,
@@ -638,6 +940,7 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   st = group_obtain(g, lp.head_words * sizeof(uint32_t), &hb);
   if (st != ACPP_SUCCESS) return st;
   lp.heads = static_cast<uint32_t *>(hb);
+  if (reduce) return enqueue_reduce<T, Op>(g, lp, stream, in, out, offsets, num_segments, total, init, num_tiles, name);
   using SIO = seg_status<typename elem_traits<T>::work>;
   static_assert(SIO::words <= seg_stride_words, "status slot too small");
   lp.region_used = sizeof(sync_header) + num_tiles * seg_stride_words * sizeof(uint64_t);
@@ -651,7 +954,7 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
     if (st != ACPP_SUCCESS) return st;
     lp.rs->mode = region_state::graph;
   } else {
-    if (lp.rs->mode == region_state::graph || lp.rs->epoch >= seg_max_epoch) {
+    if (lp.rs->mode == region_state::graph || lp.rs->epoch >= max_epoch) {
       st = fill_zero(lp.region, lp.region_cap, stream);
       if (st != ACPP_SUCCESS) return st;
       lp.rs->epoch = 0;

@@ -11,7 +11,7 @@ import pytest
 
 import opensycl_amd as acpp
 import oracle as O
-from gpu_util import ALL_OPS, INT_TYPES, dev, host, op_input, rand
+from gpu_util import ALL_OPS, INT_TYPES, dev, host, lognormal_offsets, op_input, rand
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +74,7 @@ def test_scan_launch_configs(gpu, q, scratch, cfg):
             x = np.round(x * 100).astype(np.float32)  # small integers: fp sums are exact
         got = _scan_case(torch, q, scratch, x, acpp.PLUS, "exclusive", dtype(1), config=cfg)
         assert np.array_equal(got, O.exclusive_scan(x, dtype(1))), (dtype, cfg)
+    assert scratch.watchdog_trips() == 0
 
 
 @pytest.mark.parametrize("offset", [1, 2, 3])
@@ -143,3 +144,37 @@ def test_scan_large_int32_exact(gpu, q, scratch):
     y = host(yd)
     del yd
     assert np.array_equal(y, O.exclusive_scan(x, np.int32(0)))
+
+
+def test_look_back_kernels_share_scratch_regions(gpu, q):
+    # Scans, copy_if and segmented scans draw their status regions from one
+    # cache and continue each region's epoch count.  Stale words of one kernel
+    # must never read as live to another (all use one tag layout), and no
+    # look-back may give up waiting.  Small tiles: many statuses per region.
+    torch, cache = gpu
+    off = lognormal_offsets(2000, seed=3, max_len=1 << 17)
+    ns = int(off[-1])
+    xs = rand(np.int64, ns, seed=9, lo=-1000, hi=1000)
+    xsd, od = dev(torch, xs), dev(torch, off)
+    exp_seg = O.segmented_inclusive_scan(xs, off)
+    x32 = rand(np.int32, 3 * 1024 * 1024 + 17, seed=4)
+    x32d = dev(torch, x32)
+    exp_scan = O.exclusive_scan(x32, np.int32(1))
+    exp_sel, exp_k = O.copy_if(x32, O.PRED_GT, 0)
+    for rep in range(4):
+        for cfg_seg, cfg_scan in (((64, 2), (128, 4)), ((128, 2), (64, 2)), ((256, 4, 0, 1), (128, 4, 0, 1))):
+            g = acpp.AllocationGroup(cache, 0)
+            ys = torch.zeros_like(xsd)
+            acpp.segmented_inclusive_scan(q, g, xsd, ys, od, acpp.PLUS, config=cfg_seg)
+            y32 = torch.empty_like(x32d)
+            acpp.exclusive_scan(q, g, x32d, y32, np.int32(1), acpp.PLUS, config=cfg_scan)
+            sel = torch.empty_like(x32d)
+            cnt = torch.zeros(1, dtype=torch.uint64, device="cuda")
+            acpp.copy_if(q, g, x32d, sel, acpp.PRED_GT, 0, num_copied=cnt)
+            q.wait()
+            assert g.watchdog_trips() == 0, (rep, cfg_seg, cfg_scan)
+            assert np.array_equal(host(ys), exp_seg), (rep, cfg_seg)
+            assert np.array_equal(host(y32), exp_scan), (rep, cfg_scan)
+            k = int(host(cnt)[0])
+            assert k == exp_k and np.array_equal(host(sel)[:k], exp_sel), rep
+            g.close()

@@ -83,10 +83,13 @@ def test_segmented_reduce(gpu, q, scratch, dtype):
                 assert np.array_equal(got, O.segmented_reduce(x, off, init, op)), (name, op)
 
 
-# flags: 1 = work-group-wide look-back window, 4 = one work-group per tile
+# flags: 1 = work-group-wide look-back window, 4 = one work-group per tile.
+# (64, 2) gives 256-element int64 tiles: thousands of single-wave tiles in
+# flight, the heaviest look-back contention (a regression case for the
+# look-back backoff; without it the look-backs ran out of time).
 @pytest.mark.parametrize("cfg", [(64, 2), (128, 4), (256, 2), (512, 4), (1024, 2), (1024, 4),
                                  (256, 4, 0, 1), (256, 4, 0, 4), (512, 8, 0, 5), (64, 2, 0, 4),
-                                 (1024, 8, 0, 0)])
+                                 (1024, 8, 0, 0), (64, 2, 0, 1)])
 def test_segmented_configs(gpu, q, scratch, cfg):
     torch, _ = gpu
     off = lognormal_offsets(2000, seed=3, max_len=1 << 17)
@@ -98,6 +101,7 @@ def test_segmented_configs(gpu, q, scratch, cfg):
     out = torch.zeros(len(off) - 1, dtype=torch.int64, device="cuda")
     acpp.segmented_reduce(q, scratch, xd, od, out, np.int64(0), acpp.PLUS, config=cfg)
     q.wait()
+    assert scratch.watchdog_trips() == 0
     assert np.array_equal(host(yd), O.segmented_inclusive_scan(x, off))
     assert np.array_equal(host(out), O.segmented_reduce(x, off, np.int64(0)))
 
