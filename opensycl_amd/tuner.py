@@ -45,8 +45,21 @@ DEFAULT_MODEL = os.path.join(DATA_DIR, "cost_model.json")
 WORK_GROUP_SIZES = (64, 128, 256, 512, 1024)
 ROWS = (2, 4, 8)
 CONFIGS = [(wg, r) for wg in WORK_GROUP_SIZES for r in ROWS]
-# The library's own choice when no configuration is given (segmented.hip).
+# The single fixed tile the library used before its size rule, kept as a
+# comparison point.
 DEFAULT_CONFIG = (512, 4)
+
+
+def default_config(kind: str, n: int) -> tuple:
+    """The library's own choice when no configuration is given: the size
+    rule of segmented.hip (default_work_group / default_rows)."""
+    if n < (1 << 16):
+        return (128, 2) if kind == "reduce" else (256, 2)
+    if n < (1 << 20):
+        return (256, 2) if kind == "reduce" else (1024, 2)
+    if kind == "reduce" and n < (1 << 24):
+        return (1024, 2)
+    return (1024, 4)
 
 KINDS = ("scan", "reduce")
 DTYPES = {"i32": 4, "i64": 8, "f32": 4, "f64": 8}
@@ -362,24 +375,30 @@ class NativeTuner:
 
 def regret(db: list, picks: list) -> dict:
     """Regret t(pick)/t(best) - 1 over the records, with the same statistic
-    for the library's default configuration (the method without the cost
-    model) and the mean speed-up of the picks over the default."""
-    r, rd, gain = [], [], []
+    for the library's default (its size rule, default_config: the method
+    without the cost model) and for the former single fixed tile
+    (DEFAULT_CONFIG), and the mean speed-up of the picks over the default."""
+    r, rd, rf, gain = [], [], [], []
     for rec, pick in zip(db, picks):
         t = rec["times_us"]
         best = min(t.values())
         tp = t[_key(pick)]
-        td = t[_key(DEFAULT_CONFIG)]
+        sh = rec.get("shape", {})
+        dflt = default_config(sh["kind"], sh["n"]) if "kind" in sh and "n" in sh else DEFAULT_CONFIG
+        td = t[_key(dflt)]
+        tf = t[_key(DEFAULT_CONFIG)]
         r.append(tp / best - 1.0)
         rd.append(td / best - 1.0)
+        rf.append(tf / best - 1.0)
         gain.append(td / tp)
-    r, rd, gain = np.array(r), np.array(rd), np.array(gain)
+    r, rd, rf, gain = np.array(r), np.array(rd), np.array(rf), np.array(gain)
     return {
         "shapes": len(r),
         "mean_regret": float(r.mean()), "median_regret": float(np.median(r)),
         "p90_regret": float(np.quantile(r, 0.9)), "max_regret": float(r.max()),
         "default_mean_regret": float(rd.mean()), "default_median_regret": float(np.median(rd)),
         "default_p90_regret": float(np.quantile(rd, 0.9)),
+        "fixed_512x4_mean_regret": float(rf.mean()),
         "geomean_speedup_vs_default": float(np.exp(np.log(gain).mean())),
         "exact_best_fraction": float(np.mean(r < 1e-12)),
     }

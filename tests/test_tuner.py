@@ -95,3 +95,18 @@ def test_committed_model_regret_on_held_out_shapes():
     # the learned choice beats the library's fixed default on unseen shapes
     assert r["mean_regret"] < r["default_mean_regret"], r
     assert r["geomean_speedup_vs_default"] > 1.0, r
+
+
+def test_library_default_rule_on_committed_timings():
+    # The size rule the library applies without a configuration
+    # (segmented.hip default_work_group / default_rows) against the best of
+    # the 15 configurations on the committed MI355X timings: held-out mean
+    # regret below 10 %, and better than the former single fixed tile.
+    db = T.load_db(os.path.join(T.DATA_DIR, "segmented_timings.json"))
+    held = db[400:]
+    picks = [T.default_config(r["shape"]["kind"], r["shape"]["n"]) for r in held]
+    r = T.regret(held, picks)
+    assert r["mean_regret"] < 0.10, r
+    assert r["mean_regret"] < r["fixed_512x4_mean_regret"], r
+    assert T.default_config("scan", 1 << 28) == (1024, 4)
+    assert T.default_config("reduce", 1000) == (128, 2)
