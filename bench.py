@@ -456,7 +456,32 @@ def extras_leg(acpp, q, cache, log2n):
             "gbs": round((4 * n + 4 * c) / (avg * 1e-6) / 1e9, 1),
             "count_matches_torch": c == int(sel.sum().item()),
             "output_matches_torch": bool(torch.equal(di[:c], xi[sel]))}
-        del xi, di, sel
+        # transform forms on the same input: sum of squares (unary
+        # transform_reduce) and the inclusive scan of -x
+        o32 = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+        def runt():
+            acpp.transform_reduce_unary(q, g, xi, o32, 0, acpp.PLUS, acpp.SQUARE)
+        runt()
+        g.release()
+        avg, _ = timed(q, runt, 10, release=g.release)
+        sq = int((xi.to(torch.int64) * xi.to(torch.int64)).sum().item())
+        sq32 = ((sq + (1 << 31)) % (1 << 32)) - (1 << 31)
+        res["transform_reduce_i32_square"] = {
+            "avg_us": round(avg, 2), "algorithmic_bytes": 4 * n,
+            "gbs": round(4 * n / (avg * 1e-6) / 1e9, 1),
+            "matches_torch_mod_2_32": int(o32.item()) == sq32}
+
+        def runs():
+            acpp.transform_inclusive_scan(q, g, xi, di, acpp.PLUS, acpp.NEGATE)
+        runs()
+        g.release()
+        avg, _ = timed(q, runs, 10, release=g.release)
+        res["transform_inclusive_scan_i32_negate"] = {
+            "avg_us": round(avg, 2), "algorithmic_bytes": 8 * n,
+            "gbs": round(8 * n / (avg * 1e-6) / 1e9, 1),
+            "matches_torch": bool(torch.equal(di, torch.cumsum(-xi, 0, dtype=torch.int32)))}
+        del xi, di, sel, o32
         vals = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32, device="cuda", generator=gen)
         tmax = {}
         for strat_name, strat in (("privatized", acpp.ATOMICS_PRIVATIZED), ("global", acpp.ATOMICS_GLOBAL)):
