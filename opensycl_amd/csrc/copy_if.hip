@@ -96,7 +96,8 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
   const int lane = tid & 63;
 
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_wave[R][scan_detail::max_waves];
+  __shared__ uint32_t s_wave[R * scan_detail::max_waves];  // per (row, wave): counts, then offsets
+  __shared__ uint32_t s_agg;
   __shared__ uint64_t s_excl;
   extern __shared__ unsigned char s_stage_raw[];  // the compacted tile: B * R * VEC elements
   T *s_stage = reinterpret_cast<T *>(s_stage_raw);
@@ -149,28 +150,19 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
   for (int r = 0; r < R; ++r) {
     const uint32_t incl = wave::inclusive_scan<CountOp>(static_cast<uint32_t>(__builtin_popcount(fm[r])));
     wexcl[r] = wave::shift_up<CountOp>(incl);
-    if (lane == 63) s_wave[r][w] = incl;
+    if (lane == 63) s_wave[r * nw + w] = incl;
   }
   __syncthreads();
-  uint32_t before[R];
-  uint32_t carry = 0;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    before[r] = 0;
-    for (int ww = 0; ww < nw; ++ww) {
-      const uint32_t x = s_wave[r][ww];
-      if (ww == w) before[r] = carry;
-      carry += x;
-    }
-  }
-  const uint64_t agg = carry;
-
-  // Output offset of the tile.
-  uint64_t tile_prefix = 0;
-  if (tile == 0) {
-    if (tid == 0 && p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, agg);
-  } else {
-    if (w == 0) {
+  // Wave 0 turns the (row, wave) counts into offsets, publishes the tile's
+  // count at once and looks back for the tile's output offset.
+  if (w == 0) {
+    const uint64_t agg = scan_detail::tile_prefixes<CountOp, uint32_t>(R * nw, s_wave);
+    if (tile == 0) {
+      if (lane == 0) {
+        if (p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, agg);
+        s_excl = 0;
+      }
+    } else {
       if (lane == 0) SIO::publish(p.status, tile, p.epoch, st_aggregate, agg);
       const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t>(
           p.status, tile, p.epoch, &p.sync->watchdog_trips);
@@ -179,9 +171,14 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
         s_excl = e;
       }
     }
-    __syncthreads();
-    tile_prefix = s_excl;
+    if (lane == 0) s_agg = static_cast<uint32_t>(agg);
   }
+  __syncthreads();
+  uint32_t before[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) before[r] = s_wave[r * nw + w];
+  const uint64_t agg = s_agg;
+  const uint64_t tile_prefix = s_excl;
   if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + agg;
 
   // Compact the tile's selected elements in LDS, then store the tile's

@@ -151,6 +151,29 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
   }
 }
 
+// Exclusive prefixes, in element order, of a tile's per-(row, wave) totals:
+// s_tot[r * nw + w] holds the totals on entry and the exclusive prefixes on
+// return (P = R * nw <= 128 entries).  Called by one whole wave, which scans
+// them with DPP (two entries per lane above 64); returns the tile aggregate,
+// uniform across the wave.  This replaces a walk over all R * nw totals by
+// every work-item (128 dependent LDS reads and combines per work-item for
+// 1024 x 8 tiles), which also delayed the aggregate's publication.
+template <class Op, class W>
+__device__ W tile_prefixes(int P, W *s_tot) {
+  const int lane = wave::lane_id();
+  const int per = P > 64 ? 2 : 1;
+  const int a = lane * per;
+  W v0 = Op::identity(), v1 = Op::identity();
+  if (a < P) v0 = s_tot[a];
+  if (per == 2 && a + 1 < P) v1 = s_tot[a + 1];
+  const W incl = wave::inclusive_scan<Op>(Op::apply(v0, v1));
+  const W excl = wave::shift_up<Op>(incl);
+  const W agg = wave::read_lane(incl, 63);
+  if (a < P) s_tot[a] = excl;
+  if (per == 2 && a + 1 < P) s_tot[a + 1] = Op::apply(excl, v0);
+  return agg;
+}
+
 // Exclusive prefix of `tile` with a look-back window as wide as the whole
 // work-group: work-item t polls tile (tile - 1 - t), so one round trip to
 // memory covers blockDim.x predecessors.  The number of tiles a look-back
@@ -270,8 +293,9 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   const int lane = tid & 63;
 
   __shared__ uint32_t s_tile;
-  __shared__ W s_wave[R][max_waves];
+  __shared__ W s_wave[R * max_waves];  // per (row, wave): totals, then exclusive offsets
   __shared__ W s_excl;
+  __shared__ W s_agg;
   __shared__ uint64_t s_masks[2 * max_waves];
   __shared__ W s_red[max_waves];
 
@@ -336,32 +360,23 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   for (int r = 0; r < R; ++r) {
     const W incl = wave::inclusive_scan<Op>(v[r][VEC - 1]);
     wexcl[r] = wave::shift_up<Op>(incl);
-    if (lane == 63) s_wave[r][w] = incl;
+    if (lane == 63) s_wave[r * nw + w] = incl;
   }
   __syncthreads();
-  // Offsets of (row, wave) inside the tile, in element order.
-  W before[R];
-  W carry = Op::identity();
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    before[r] = Op::identity();
-    for (int ww = 0; ww < nw; ++ww) {
-      const W x = s_wave[r][ww];
-      if (ww == w) before[r] = carry;
-      carry = Op::apply(carry, x);
-    }
-  }
-  const W agg = carry;
-
-  W tile_prefix;
-  if (tile == 0) {
-    tile_prefix = p.has_init ? p.init : Op::identity();
-    if (tid == 0 && p.num_tiles > 1)
-      SIO::publish(p.status, 0, epoch, st_prefix, Op::apply(tile_prefix, agg));
-  } else {
-    if (tid == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
-    if (!p.block_lookback) {
-      if (w == 0) {
+  // Wave 0 turns the (row, wave) totals into offsets and publishes the
+  // tile's aggregate at once; with the one-wave look-back it then looks
+  // back, while the other waves wait at the barrier.
+  if (w == 0) {
+    const W agg = tile_prefixes<Op, W>(R * nw, s_wave);
+    if (tile == 0) {
+      const W first = p.has_init ? p.init : Op::identity();
+      if (lane == 0) {
+        if (p.num_tiles > 1) SIO::publish(p.status, 0, epoch, st_prefix, Op::apply(first, agg));
+        s_excl = first;
+      }
+    } else {
+      if (lane == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
+      if (!p.block_lookback) {
         const W excl = lookback<Op, W>(p.status, tile, epoch, &p.sync->watchdog_trips);
         if (lane == 0) {
           if (tile + 1 < p.num_tiles)
@@ -369,14 +384,24 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
           s_excl = excl;
         }
       }
-      __syncthreads();
-      tile_prefix = s_excl;
-    } else {
-      tile_prefix = block_lookback<Op, W>(p.status, tile, epoch, s_masks, s_red,
-                                         &p.sync->watchdog_trips);
-      if (tid == 0 && tile + 1 < p.num_tiles)
-        SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(tile_prefix, agg));
     }
+    if (lane == 0) s_agg = agg;
+  }
+  __syncthreads();
+  // Offsets of (row, wave) inside the tile, in element order.
+  W before[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) before[r] = s_wave[r * nw + w];
+
+  W tile_prefix;
+  if (tile != 0 && p.block_lookback) {
+    const W agg = s_agg;
+    tile_prefix = block_lookback<Op, W>(p.status, tile, epoch, s_masks, s_red,
+                                       &p.sync->watchdog_trips);
+    if (tid == 0 && tile + 1 < p.num_tiles)
+      SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(tile_prefix, agg));
+  } else {
+    tile_prefix = s_excl;
   }
 
 #pragma unroll
