@@ -153,37 +153,36 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
     if (lane == 63) s_wave[r * nw + w] = incl;
   }
   __syncthreads();
-  // Wave 0 turns the (row, wave) counts into offsets, publishes the tile's
-  // count at once and looks back for the tile's output offset.
+  // Wave 0 turns the (row, wave) counts into offsets and publishes the
+  // tile's count at once.
   if (w == 0) {
-    const uint64_t agg = scan_detail::tile_prefixes<CountOp, uint32_t>(R * nw, s_wave);
-    if (tile == 0) {
-      if (lane == 0) {
-        if (p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, agg);
+    const uint32_t agg0 = scan_detail::tile_prefixes<CountOp, uint32_t>(R * nw, s_wave);
+    if (lane == 0) {
+      if (tile == 0) {
+        if (p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, static_cast<uint64_t>(agg0));
         s_excl = 0;
+      } else {
+        SIO::publish(p.status, tile, p.epoch, st_aggregate, static_cast<uint64_t>(agg0));
       }
-    } else {
-      if (lane == 0) SIO::publish(p.status, tile, p.epoch, st_aggregate, agg);
-      const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t>(
-          p.status, tile, p.epoch, &p.sync->watchdog_trips);
-      if (lane == 0) {
-        if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, p.epoch, st_prefix, e + agg);
-        s_excl = e;
-      }
+      s_agg = agg0;
     }
-    if (lane == 0) s_agg = static_cast<uint32_t>(agg);
   }
   __syncthreads();
   uint32_t before[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) before[r] = s_wave[r * nw + w];
   const uint64_t agg = s_agg;
-  const uint64_t tile_prefix = s_excl;
-  if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + agg;
-
-  // Compact the tile's selected elements in LDS, then store the tile's
-  // output range with consecutive work-items on consecutive elements (fully
-  // coalesced whatever the selection pattern; one barrier per tile).
+  // Wave 0 looks back for the tile's output offset; the compaction of the
+  // tile's selected elements into LDS needs only positions inside the tile,
+  // so every wave does its share meanwhile (wave 0 after its look-back).
+  if (w == 0 && tile != 0) {
+    const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t>(
+        p.status, tile, p.epoch, &p.sync->watchdog_trips);
+    if (lane == 0) {
+      if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, p.epoch, st_prefix, e + agg);
+      s_excl = e;
+    }
+  }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     uint32_t pos = before[r] + wexcl[r];
@@ -194,6 +193,10 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
     }
   }
   __syncthreads();
+  // Store the tile's output range from LDS with consecutive work-items on
+  // consecutive elements (fully coalesced whatever the selection pattern).
+  const uint64_t tile_prefix = s_excl;
+  if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + agg;
   T *dst = p.out + tile_prefix;
   const uint32_t count = static_cast<uint32_t>(agg);
   if (!p.out_vec) {
