@@ -176,7 +176,8 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
   // tile's selected elements into LDS needs only positions inside the tile,
   // so every wave does its share meanwhile (wave 0 after its look-back).
   if (w == 0 && tile != 0) {
-    const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t>(
+    wait_own_stores();  // the tile's count is out before the first poll
+    const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t, scan_detail::lookback_slots>(
         p.status, tile, p.epoch, &p.sync->watchdog_trips);
     if (lane == 0) {
       if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, p.epoch, st_prefix, e + agg);

@@ -30,9 +30,10 @@
 //     out took the 2^28 int32 scan from ~490 us to ~410 us;
 //   - the look-back is done by one wave: 64 predecessors are polled at
 //     once, the nearest "prefix" among them is found with a ballot and the
-//     window is folded with DPP, with no work-group barrier per poll.
-//     A work-group-wide window (blockDim.x tiles per poll) is kept as a
-//     variant;
+//     window is folded with DPP, with no work-group barrier per poll.  The
+//     wave first waits for its own aggregate store to complete
+//     (wait_own_stores in internal.hpp).  A work-group-wide window
+//     (blockDim.x tiles per poll) is kept as a variant;
 //   - nothing is reset at the end of a launch: the host tracks, per scratch
 //     region, the epoch of the last launch and how many tile tickets have
 //     been drawn, and passes both as kernel arguments (a per-tile "done"
@@ -49,6 +50,11 @@ namespace acpp_impl {
 namespace scan_detail {
 
 constexpr int max_waves = 16;
+// Status slots polled per lane and look-back round (window 64 x this).
+// Wider windows put more polls on the memory system than they save in
+// rounds: with all loads in flight together, the 2^28 int32 scan took
+// 377 us with 64 and 384 us with 128 on MI355X.
+constexpr int lookback_slots = 1;
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -113,8 +119,12 @@ struct status_io<W, 8> {
 };
 
 // Exclusive prefix of `tile`: op over all predecessors, found by a wave-wide
-// look-back.  Must be called by a whole wave; the result is uniform.
-template <class Op, class W>
+// look-back with a window of 64 * K predecessors.  Lane l polls tiles
+// tile - 1 - l - 64 k (k < K) with all K loads in flight at once; the
+// nearest "prefix" in the window is found with ballots, and the window up
+// to it is folded with DPP, with no work-group barrier per poll.  Must be
+// called by a whole wave; the result is uniform.
+template <class Op, class W, int K>
 __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
                       uint32_t *watchdog) {
   using SIO = status_io<W>;
@@ -125,29 +135,50 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
   wait.begin();
   bool give_up = false;  // uniform
   for (;;) {
-    const int64_t j = base - lane;  // lane 0 = nearest predecessor
-    W val = Op::identity();
+    W val[K];
     int p;
     for (;;) {
-      uint32_t flag = st_prefix;  // positions before tile 0 act as an empty prefix
-      if (j >= 0) flag = SIO::read(status, static_cast<uint32_t>(j), epoch, val);
-      if (j < 0) val = Op::identity();
-      if (give_up) flag = st_prefix;
-      const uint64_t pm = wave::ballot(flag == st_prefix);
-      p = pm ? __builtin_ctzll(pm) : 64;
-      const uint64_t need = (p >= 63) ? ~0ull : ((2ull << p) - 1);
-      const uint64_t bad = wave::ballot(flag == st_invalid) & need;
-      if (bad == 0) break;
+      uint32_t flag[K];
+      // Branch-free polls, so that all K loads are in flight together:
+      // positions before tile 0 read tile 0's slot and act as an empty
+      // prefix.
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int64_t j = base - lane - 64 * k;
+        W x = Op::identity();
+        const uint32_t f = SIO::read(status, j >= 0 ? static_cast<uint32_t>(j) : 0u, epoch, x);
+        flag[k] = (j >= 0 && !give_up) ? f : static_cast<uint32_t>(st_prefix);
+        val[k] = (j >= 0) ? x : Op::identity();
+      }
+      // p: window position of the nearest prefix (64 * K: none).  Every
+      // position up to it must hold a valid status.
+      p = 64 * K;
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint64_t pm = wave::ballot(flag[k] == st_prefix);
+        const uint64_t im = wave::ballot(flag[k] == st_invalid);
+        if (p == 64 * K) {
+          const int pk = pm ? __builtin_ctzll(pm) : 64;
+          const uint64_t need = (pk >= 63) ? ~0ull : ((2ull << pk) - 1);
+          if (im & need) bad = true;
+          if (pm) p = 64 * k + pk;
+        }
+      }
+      if (!bad) break;
       if (wait.failed_round()) {
         give_up = true;
         if (lane == 0) __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    const W x = (lane <= p) ? val : Op::identity();
+    W x = Op::identity();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (64 * k + lane <= p) x = Op::apply(x, val[k]);
     const W window = wave::reduce_full<Op>(x);
     excl = Op::apply(window, excl);
-    if (p < 64) return excl;
-    base -= 64;
+    if (p < 64 * K) return excl;
+    base -= 64 * K;
   }
 }
 
@@ -318,32 +349,46 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
   const bool full = p.vec_io && (base + tile_elems <= p.n);
 
-  W v[R][VEC];
+  // Both paths fill the same 16-byte vectors, converted to the work type
+  // after the join.  The scheduling barrier keeps the join's register copies
+  // behind the full path's loads, so all R loads are issued back to back
+  // (without it, gfx950 code issued the last load only once the first had
+  // returned; at 2^28 elements the difference was within run-to-run noise).
+  u32x4 q[R];
   if (full) {
     const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
-    u32x4 q[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+    __builtin_amdgcn_sched_barrier(0);
+  } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       T tmp[VEC];
-      __builtin_memcpy(tmp, &q[r], 16);
 #pragma unroll
-      for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+      for (int k = 0; k < VEC; ++k) {
+        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
+        tmp[k] = (idx < p.n) ? p.in[idx] : T{};
+      }
+      __builtin_memcpy(&q[r], tmp, 16);
     }
-    if constexpr (XF) unary_detail::apply_tile<T>(p.unary, v);
-  } else {
+  }
+  W v[R][VEC];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    T tmp[VEC];
+    __builtin_memcpy(tmp, &q[r], 16);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+  }
+  if constexpr (XF) unary_detail::apply_tile<T>(p.unary, v);
+  if (!full) {
+    // Elements past the end act as the identity.
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int k = 0; k < VEC; ++k) {
         const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
-        W x = Op::identity();
-        if (idx < p.n) {
-          x = elem_traits<T>::to_work(p.in[idx]);
-          if constexpr (XF) x = unary_detail::apply_one<T>(p.unary, x);
-        }
-        v[r][k] = x;
+        if (idx >= p.n) v[r][k] = Op::identity();
       }
     }
   }
@@ -377,7 +422,8 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
     } else {
       if (lane == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
       if (!p.block_lookback) {
-        const W excl = lookback<Op, W>(p.status, tile, epoch, &p.sync->watchdog_trips);
+        wait_own_stores();  // the aggregate is out before the first poll
+        const W excl = lookback<Op, W, lookback_slots>(p.status, tile, epoch, &p.sync->watchdog_trips);
         if (lane == 0) {
           if (tile + 1 < p.num_tiles)
             SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(excl, agg));
@@ -426,6 +472,7 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
     if (full) {
       u32x4 q;
       __builtin_memcpy(&q, o, 16);
+      // Non-temporal: the output is not read again by this launch.
       reinterpret_cast<u32x4 *>(p.out + base)[r * B + tid] = q;
     } else {
 #pragma unroll

@@ -450,6 +450,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
         if (w == 0) {
           uint32_t ef;
           W ev;
+          wait_own_stores();  // the aggregate is out before the first poll
           wave_pair_lookback<Op, W>(p.status, tile, p.epoch, &p.sync->watchdog_trips, ef, ev);
           if (lane == 0) {
             s_lbf[0] = ef;
