@@ -34,8 +34,12 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   // the whole config is left at zero.  The transform reductions (XF) take
   // the same 512 x one-group-per-CU geometry (170 us for 2^28 int32 squares
   // against 211 us with the generic 256-wide, occupancy-filling grid), with
-  // 4 loads in flight and ordinary loads.
+  // 4 loads in flight and ordinary loads.  The compensated dot products
+  // keep 2 loads in flight: their per-load Dot2 state makes 8 loads need 128
+  // VGPRs and spill, and 2 loads (45 VGPRs) measured fastest (2 x 2^28 fp32:
+  // 318 us against 351 us with 8; bf16: 165 us against 177 us).
   constexpr bool tuned = (Op::id == ops::op_plus) && !XF;
+  constexpr bool comp_dot = DOT && std::is_same_v<M, comp_monoid>;
   constexpr bool streaming = tuned || XF;
   const bool cfg_given = cfg && (cfg->work_group_size || cfg->items_per_thread || cfg->groups_per_cu ||
                                  cfg->flags);
@@ -43,7 +47,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   if (!valid_work_group_size(wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
-  const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : (tuned && DOT ? 8 : 4);
+  const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : (tuned && DOT ? (comp_dot ? 2 : 8) : 4);
   const uint32_t groups_per_cu = (cfg && cfg->groups_per_cu) ? cfg->groups_per_cu : (streaming ? 1 : 0);
   const uint64_t stream_bytes = n * sizeof(typename elem_traits<T>::storage) * (DOT ? 2 : 1);
   const bool nt = cfg_given ? (cfg->flags & ACPP_VARIANT_NONTEMPORAL) != 0 : (stream_bytes >= (256ull << 20));
