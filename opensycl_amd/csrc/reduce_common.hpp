@@ -199,7 +199,19 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
       qa[u] = load16<NT>(va, c + u * stride);
       if constexpr (DOT) qb[u] = load16<NT>(vb, c + u * stride);
     }
-    if constexpr (XF) {
+    if constexpr (XF && VEC > 4) {
+      // Narrow elements (8 or 16 per load): one uniform switch per row.  A
+      // whole batch of 8-bit elements widened at once (64 values) spilled.
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        W xa[VEC];
+        unpack<T, W, VEC>(qa[u], xa);
+        unary_detail::apply_row<T>(p.unary, xa);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc[u] = M::insert(acc[u], xa[k]);
+      }
+      continue;
+    } else if constexpr (XF) {
       // One uniform switch for the whole batch.
       W xt[UNROLL][VEC];
 #pragma unroll
