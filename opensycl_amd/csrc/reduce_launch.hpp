@@ -25,30 +25,31 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   acpp_status st = enter_device(g, &cus);
   if (st != ACPP_SUCCESS) return st;
 
-  // Library defaults.  For sums and dot products (the tuned path) they are
-  // the configurations measured fastest on MI355X at 2^28 elements: one
-  // resident work-group per CU (512 work-items x 4 loads in flight for a
-  // single operand, 1024 x 8 for a dot product) streaming with
-  // non-temporal loads once the input is larger than the 256 MiB
-  // last-level cache.  The non-temporal choice is the library's only when
-  // the whole config is left at zero.  The transform reductions (XF) take
-  // the same 512 x one-group-per-CU geometry (170 us for 2^28 int32 squares
-  // against 211 us with the generic 256-wide, occupancy-filling grid), with
-  // 4 loads in flight and ordinary loads.  The compensated dot products
-  // keep 2 loads in flight: their per-load Dot2 state makes 8 loads need 128
-  // VGPRs and spill, and 2 loads (45 VGPRs) measured fastest (2 x 2^28 fp32:
-  // 318 us against 351 us with 8; bf16: 165 us against 177 us).
+  // Library defaults, measured on MI355X at 2^28 elements.  Every
+  // reduction streams with one resident work-group per CU: 512 work-items x
+  // 4 loads in flight for a single operand, 1024 x 8 for a dot product.  The
+  // generic 256-wide, occupancy-filling grid was slower for every operator
+  // (int32 max: 213 us against 170 us; int32 squares through the transform
+  // form: 211 us against 170 us).  1-byte elements take 1024 work-items: 16
+  // elements per load make the per-element arithmetic the limit, and 512
+  // work-items left 2^30 int8 at 301 us against 190 us.  Sums (the tuned
+  // path) stream with non-temporal loads once the input is larger than the
+  // 256 MiB last-level cache; that choice is the library's only when the
+  // whole config is left at zero.  The compensated dot products keep 2 loads
+  // in flight: their per-load Dot2 state makes 8 loads need 128 VGPRs and
+  // spill, and 2 loads (45 VGPRs) measured fastest (2 x 2^28 fp32: 318 us
+  // against 351 us with 8; bf16: 165 us against 177 us).
   constexpr bool tuned = (Op::id == ops::op_plus) && !XF;
   constexpr bool comp_dot = DOT && std::is_same_v<M, comp_monoid>;
-  constexpr bool streaming = tuned || XF;
+  constexpr bool narrow8 = sizeof(typename elem_traits<T>::storage) == 1;
   const bool cfg_given = cfg && (cfg->work_group_size || cfg->items_per_thread || cfg->groups_per_cu ||
                                  cfg->flags);
-  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : (streaming ? (DOT ? 1024 : 512) : 256);
+  const uint32_t wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : ((DOT || narrow8) ? 1024 : 512);
   if (!valid_work_group_size(wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
   const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : (tuned && DOT ? (comp_dot ? 2 : 8) : 4);
-  const uint32_t groups_per_cu = (cfg && cfg->groups_per_cu) ? cfg->groups_per_cu : (streaming ? 1 : 0);
+  const uint32_t groups_per_cu = (cfg && cfg->groups_per_cu) ? cfg->groups_per_cu : 1;
   const uint64_t stream_bytes = n * sizeof(typename elem_traits<T>::storage) * (DOT ? 2 : 1);
   const bool nt = cfg_given ? (cfg->flags & ACPP_VARIANT_NONTEMPORAL) != 0 : (stream_bytes >= (256ull << 20));
   if (unroll != 2 && unroll != 4 && unroll != 8 && (unroll != 16 || DOT))

@@ -380,7 +380,14 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
 #pragma unroll
     for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
   }
-  if constexpr (XF) unary_detail::apply_tile<T>(p.unary, v);
+  if constexpr (XF && VEC > 4) {
+    // Narrow elements: one uniform switch per row (a whole tile of widened
+    // 8- or 16-bit elements under one switch spilled to scratch).
+#pragma unroll
+    for (int r = 0; r < R; ++r) unary_detail::apply_row<T>(p.unary, v[r]);
+  } else if constexpr (XF) {
+    unary_detail::apply_tile<T>(p.unary, v);
+  }
   if (!full) {
     // Elements past the end act as the identity.
 #pragma unroll
