@@ -399,6 +399,9 @@ def atomic_fetch_strided(q: Queue, scratch: AllocationGroup, op: str, values, ta
 PRED_NONZERO, PRED_EQ, PRED_NE, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_MASK_EQ = range(8)
 
 
+_COPY_IF_TAGS = ("i32", "u32", "i64", "u64", "f32", "f64")
+
+
 def copy_if(q: Queue, scratch: AllocationGroup, x, d, pred: int, operand=0, mask=0, num_copied=None,
             config=None) -> None:
     """algorithms::copy_if (algorithm.hpp:178-243) with a data predicate:
@@ -408,6 +411,8 @@ def copy_if(q: Queue, scratch: AllocationGroup, x, d, pred: int, operand=0, mask
     if num_copied is not None and _tag(num_copied.dtype) != "u64":
         raise TypeError("num_copied must be a uint64 tensor")
     tag = _tag(x.dtype)
+    if tag not in _COPY_IF_TAGS:
+        raise TypeError(f"copy_if takes 4- or 8-byte elements (i32, u32, i64, u64, f32, f64), not {tag}")
     fn = getattr(_abi.load(), f"acpp_copy_if_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), int(pred), _scalar(tag, operand),
              _scalar(tag, mask), _ptr(num_copied) if num_copied is not None else None, _cfg(config)))
@@ -421,6 +426,8 @@ def copy_if_stencil(q: Queue, scratch: AllocationGroup, x, stencil, d, num_copie
     if stencil.numel() < x.numel() or stencil.element_size() != 1:
         raise ValueError("stencil must be a one-byte tensor at least as long as x")
     tag = _tag(x.dtype)
+    if tag not in _COPY_IF_TAGS:
+        raise TypeError(f"copy_if takes 4- or 8-byte elements (i32, u32, i64, u64, f32, f64), not {tag}")
     fn = getattr(_abi.load(), f"acpp_copy_if_stencil_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(stencil), _ptr(d),
              _ptr(num_copied) if num_copied is not None else None, _cfg(config)))
