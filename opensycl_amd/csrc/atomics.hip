@@ -20,12 +20,16 @@
 //     second kernel adds their column sums to the bins, which keeps global
 //     atomics off the hot path entirely.
 //   - keys are streamed with 128-bit loads, several per work-item in flight;
-//   - strided fetch_min / fetch_max / fetch_add (slot = i mod num_slots): when
-//     num_slots divides the per-group stride each work-item always meets the
-//     same slots, so it keeps them in registers, and the group combines them
-//     once at the end;
+//   - strided fetch_min / fetch_max / fetch_add (slot = i mod num_slots):
+//     the grid is sized so that num_slots divides the grid stride whenever
+//     that fits; then each work-item always meets the same slots, keeps them
+//     in registers, and the group combines them once at the end.  Otherwise
+//     the elements go through LDS atomics, with 16-byte loads and slots
+//     tracked incrementally;
 //   - strategy "global": one device-scope atomic per element, the
 //     reference's pattern, kept for comparison.
+#include <numeric>
+
 #include "launch_util.hpp"
 
 namespace acpp_impl {
@@ -197,10 +201,11 @@ struct rmw_add {
 constexpr int scope_wg = __HIP_MEMORY_SCOPE_WORKGROUP;
 constexpr int scope_dev = __HIP_MEMORY_SCOPE_AGENT;
 
-// Register-privatised form: requires num_slots | (VEC * blockDim.x).  Then
-// element head + VEC*c + k of chunk c = gtid + m*stride lands in slot
-// (head + VEC*threadIdx.x + k) mod num_slots for every m, so each work-item
-// keeps VEC fixed slots in registers over the whole grid-stride loop.
+// Register-privatised form: requires num_slots | (VEC * total work-items),
+// which the launcher arranges through the grid size.  Then element
+// head + VEC*c + k of chunk c = gtid + m*stride lands in slot
+// (head + VEC*gtid + k) mod num_slots for every m, so each work-item keeps
+// VEC fixed slots in registers over the whole grid-stride loop.
 template <class T, class R>
 __global__ __launch_bounds__(1024) void strided_register_kernel(const T *__restrict__ v, geom gm,
                                                                 uint32_t num_slots, T *target) {
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(1024) void strided_register_kernel(const T *__restr
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < VEC; ++k)
-    R::template atomic<scope_wg>(&lds[(gm.head + VEC * threadIdx.x + k) % num_slots], acc[k]);
+    R::template atomic<scope_wg>(&lds[(gm.head + VEC * gtid + k) % num_slots], acc[k]);
   for (uint64_t s = gtid; s < gm.nscalar; s += stride) {
     const uint64_t i = scalar_index(gm, s);
     R::template atomic<scope_wg>(&lds[i % num_slots], v[i]);
@@ -247,17 +252,59 @@ __global__ __launch_bounds__(1024) void strided_register_kernel(const T *__restr
     R::template atomic<scope_dev>(&target[i], lds[i]);
 }
 
-// LDS-privatised form for any slot count that fits the LDS budget.
+// LDS-privatised form for any slot count that fits the LDS budget.  Keys
+// are streamed with 16-byte loads (4 in flight) and the slot of each
+// element is tracked incrementally: chunk c starts at slot
+// (head + VEC c) mod num_slots, and moving c on by the grid stride adds a
+// fixed step.  (One scalar load and one 64-bit modulo per element ran at a
+// sixth of the register form's rate.)
 template <class T, class R>
-__global__ __launch_bounds__(1024) void strided_lds_kernel(const T *__restrict__ v, uint64_t n,
+__global__ __launch_bounds__(1024) void strided_lds_kernel(const T *__restrict__ v, geom gm,
                                                            uint32_t num_slots, T *target) {
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int UNROLL = 4;
   extern __shared__ unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
   for (uint32_t i = threadIdx.x; i < num_slots; i += blockDim.x) lds[i] = R::identity();
   __syncthreads();
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+  const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const u32x4 *vv = reinterpret_cast<const u32x4 *>(v + gm.head);
+  // slot of the first element of chunk c, and its advance per stride
+  const uint32_t step = static_cast<uint32_t>((VEC * stride) % num_slots);
+  uint32_t s0 = static_cast<uint32_t>((gm.head + VEC * gtid) % num_slots);
+  auto advance = [&](uint32_t s, uint32_t by) {
+    s += by;  // s, by < num_slots <= 2^31
+    return s >= num_slots ? s - num_slots : s;
+  };
+  auto fold = [&](u32x4 q, uint32_t s) {
+    T e[VEC];
+    __builtin_memcpy(e, &q, 16);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      R::template atomic<scope_wg>(&lds[s], e[k]);
+      s = advance(s, 1);
+    }
+  };
+  uint64_t c = gtid;
+  for (; c + (UNROLL - 1) * stride < gm.nvec; c += UNROLL * stride) {
+    u32x4 q[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) q[u] = __builtin_nontemporal_load(vv + c + u * stride);
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      fold(q[u], s0);
+      s0 = advance(s0, step);
+    }
+  }
+  for (; c < gm.nvec; c += stride) {
+    fold(vv[c], s0);
+    s0 = advance(s0, step);
+  }
+  for (uint64_t s = gtid; s < gm.nscalar; s += stride) {
+    const uint64_t i = scalar_index(gm, s);
     R::template atomic<scope_wg>(&lds[i % num_slots], v[i]);
+  }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < num_slots; i += blockDim.x)
     R::template atomic<scope_dev>(&target[i], lds[i]);
@@ -380,14 +427,20 @@ acpp_status strided(acpp_allocation_group *g, hipStream_t stream, const T *value
     hipLaunchKernelGGL((strided_global_kernel<T, R>), dim3(grid), dim3(wg), 0, stream, values, n, num_slots, target);
   } else {
     const size_t lds = static_cast<size_t>(num_slots) * sizeof(T);
-    if (((VEC * wg) % num_slots) == 0) {
-      const geom gm = make_geom(values, n);
-      const uint32_t grid = pick_grid(gm.nvec + gm.nscalar, wg * 4, cap_groups);
-      hipLaunchKernelGGL((strided_register_kernel<T, R>), dim3(grid), dim3(wg), lds, stream, values, gm,
+    const geom gm = make_geom(values, n);
+    const uint32_t grid = pick_grid(gm.nvec + gm.nscalar, wg * 4, cap_groups);
+    // The register form needs VEC * wg * grid to be a multiple of
+    // num_slots: grid a multiple of `period`.  (1000 slots: period 125 with
+    // 256-wide groups; with per-element LDS atomics instead, fp32 adds ran
+    // at 0.8 TB/s.)
+    const uint64_t period = num_slots / std::gcd<uint64_t>(static_cast<uint64_t>(VEC) * wg, num_slots);
+    if (period <= grid) {
+      const uint32_t grid_reg = static_cast<uint32_t>((grid / period) * period);
+      hipLaunchKernelGGL((strided_register_kernel<T, R>), dim3(grid_reg), dim3(wg), lds, stream, values, gm,
                          num_slots, target);
     } else {
-      const uint32_t grid = pick_grid(n, wg * 4, cap_groups);
-      hipLaunchKernelGGL((strided_lds_kernel<T, R>), dim3(grid), dim3(wg), lds, stream, values, n, num_slots, target);
+      hipLaunchKernelGGL((strided_lds_kernel<T, R>), dim3(grid), dim3(wg), lds, stream, values, gm,
+                         num_slots, target);
     }
   }
   st = check_launch(name);

@@ -44,6 +44,7 @@ struct params {
   T mask;
   int vec_io;
   int out_vec;  // output 16-byte aligned
+  int stencil_vec;  // stencil aligned to VEC bytes: one load per 16-byte data vector
   uint32_t num_tiles;
   uint32_t epoch;
   uint32_t ticket_base;
@@ -73,6 +74,23 @@ template <int PRED, class T, int R, int VEC>
 __device__ __forceinline__ void select_bits(const params<T> &p, const T (&v)[R][VEC], uint64_t base, bool full,
                                             uint32_t (&fm)[R]) {
   const int B = blockDim.x;
+  if constexpr (PRED == ACPP_PRED_STENCIL && (VEC == 2 || VEC == 4)) {
+    if (full && p.stencil_vec) {
+      // The VEC stencil bytes of a data vector in one load.
+      using SW = std::conditional_t<VEC == 4, uint32_t, uint16_t>;
+      const SW *sw = reinterpret_cast<const SW *>(p.stencil + base);
+      SW bytes[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) bytes[r] = sw[r * B + threadIdx.x];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        fm[r] = 0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) fm[r] |= (((bytes[r] >> (8 * k)) & 0xffu) != 0u ? 1u : 0u) << k;
+      }
+      return;
+    }
+  }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     fm[r] = 0;
@@ -294,6 +312,7 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   p.num_copied = num_copied;
   p.n = n;
   p.pred = pred;
+  p.stencil_vec = stencil && (reinterpret_cast<uintptr_t>(stencil) % VEC) == 0;
   p.operand = operand;
   p.mask = mask;
   p.vec_io = (reinterpret_cast<uintptr_t>(in) & 15) == 0;

@@ -103,3 +103,22 @@ def test_fetch_max_i32_oracle(gpu, q, scratch):
     acpp.atomic_fetch_strided(q, scratch, "max", dev(torch, v)[3:], t, acpp.ATOMICS_PRIVATIZED)
     q.wait()
     assert np.array_equal(host(t), O.fetch_max_strided(v[3:], 1024))
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.float32])
+def test_strided_lds_form_misaligned_view(gpu, q, scratch, dtype):
+    # 1000 slots do not divide the work-group stride: the LDS form, with a
+    # view 3 elements into the buffer (scalar head, then 16-byte loads)
+    torch, _ = gpu
+    n = (1 << 21) + 9
+    v = np.round(rand(np.float64, n + 3, seed=5) * 4).astype(dtype)
+    vd = dev(torch, v)[3:]
+    idx = (np.arange(n) % 1000).astype(np.uint32)
+    for op in ("max", "min", "add"):
+        start = np.zeros(1000, dtype=dtype) + dtype(3)
+        t = dev(torch, start)
+        acpp.atomic_fetch_strided(q, scratch, op, vd, t, acpp.ATOMICS_PRIVATIZED)
+        q.wait()
+        exp = start.copy()
+        {"max": np.maximum, "min": np.minimum, "add": np.add}[op].at(exp, idx, v[3:])
+        assert np.array_equal(host(t), exp), op

@@ -69,10 +69,21 @@ def test_stencil_and_misaligned(gpu, q, scratch, dtype):
     out, c = _run(torch, q, scratch, x, None, stencil=st)
     ref, rc = O.copy_if(x, O.PRED_STENCIL, stencil=st)
     assert c == rc and np.array_equal(out, ref)
+    ref_st, rc_st = ref, rc
     for offset in (1, 3):  # input not 16-byte aligned: scalar loads
         out, c = _run(torch, q, scratch, x, acpp.PRED_GT, dtype(0), offset=offset)
         ref, rc = O.copy_if(x, O.PRED_GT, dtype(0))
         assert c == rc and np.array_equal(out, ref), offset
+    # stencil views: 1 byte off (byte-wise stencil reads) and 16 bytes off
+    # (one load per data vector)
+    for soff in (1, 16):
+        sd_full = dev(torch, np.concatenate([np.full(soff, 7, dtype=np.uint8), st]))
+        d = torch.empty((n,), dtype=torch.from_numpy(x[:0]).dtype, device="cuda")
+        cnt64 = torch.zeros(1, dtype=torch.int64, device="cuda")
+        acpp.copy_if_stencil(q, scratch, dev(torch, x), sd_full[soff:], d, cnt64.view(torch.uint64))
+        q.wait()
+        c = int(cnt64.item())
+        assert c == rc_st and np.array_equal(host(d)[:c], ref_st), soff
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
