@@ -133,7 +133,7 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
 
   T v[R][VEC];
   if (full) {
-    const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
+    const scan_detail::u32x4_a4 *src = reinterpret_cast<const scan_detail::u32x4_a4 *>(p.in + base);
     u32x4 q[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
@@ -315,7 +315,9 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   p.stencil_vec = stencil && (reinterpret_cast<uintptr_t>(stencil) % VEC) == 0;
   p.operand = operand;
   p.mask = mask;
-  p.vec_io = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  // 16-byte input loads need only element alignment for 4- and 8-byte
+  // elements (see scan_detail::u32x4_a4).
+  p.vec_io = (reinterpret_cast<uintptr_t>(in) & 3) == 0;
   p.out_vec = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.epoch = epoch;

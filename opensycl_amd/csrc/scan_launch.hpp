@@ -95,7 +95,10 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   p.init = init;
   p.has_init = has_init ? 1 : 0;
   p.exclusive = exclusive ? 1 : 0;
-  p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  // 16-byte vectors need 16-byte alignment for 1- and 2-byte elements, only
+  // element (4-byte) alignment for 4- and 8-byte elements.
+  p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) &
+              (sizeof(T) >= 4 ? 3u : 15u)) == 0;
   p.block_lookback = (cfg && (cfg->flags & variant_block_lookback)) ? 1 : 0;
   p.unary = unary;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
