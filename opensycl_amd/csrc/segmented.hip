@@ -38,6 +38,9 @@ namespace seg_detail {
 
 constexpr int max_waves = 16;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// 16-byte vector accesses need dword alignment only; every segmented element
+// type is 4 or 8 bytes, so any element-aligned view takes the vector path.
+typedef unsigned int u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 enum : uint32_t { st_invalid = 0, st_aggregate = 1, st_prefix = 2 };
 
@@ -346,7 +349,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
     W v[R][VEC];
     u32x4 q[R];
     if (full) {
-      const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
+      const u32x4_a4 *src = reinterpret_cast<const u32x4_a4 *>(p.in + base);
 #pragma unroll
       for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
     } else {
@@ -590,7 +593,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
         for (int k = 0; k < VEC; ++k) ot[k] = static_cast<T>(o[k]);
         u32x4 qv;
         __builtin_memcpy(&qv, ot, 16);
-        reinterpret_cast<u32x4 *>(p.out + base)[r * B + tid] = qv;
+        reinterpret_cast<u32x4_a4 *>(p.out + base)[r * B + tid] = qv;
       } else {
 #pragma unroll
         for (int k = 0; k < VEC; ++k)
@@ -659,7 +662,7 @@ __global__ __launch_bounds__(1024) void segmented_reduce_kernel(params<T> p,
   W v[R][VEC];
   u32x4 q[R];
   if (full) {
-    const u32x4 *src = reinterpret_cast<const u32x4 *>(p.in + base);
+    const u32x4_a4 *src = reinterpret_cast<const u32x4_a4 *>(p.in + base);
 #pragma unroll
     for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
   } else {
@@ -884,7 +887,7 @@ acpp_status enqueue_reduce(acpp_allocation_group *g, launch_plan<T, Op> &lp, hip
   p.total = total;
   p.init = init;
   p.reduce = 1;
-  p.vec_io = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  p.vec_io = (reinterpret_cast<uintptr_t>(in) & 3) == 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.epoch = 0;
   p.ticket_base = 0;
@@ -923,6 +926,7 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
                     const uint64_t *offsets, uint64_t num_segments, uint64_t total,
                     typename elem_traits<T>::work init, bool reduce,
                     const acpp_launch_config *cfg, const char *name) {
+  static_assert(sizeof(T) >= 4, "16-byte vector accesses at 4-byte alignment (u32x4_a4)");
   constexpr int VEC = 16 / sizeof(T);
   int cus = 0;
   acpp_status st = enter_device(g, &cus);
@@ -987,7 +991,7 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   p.total = total;
   p.init = init;
   p.reduce = reduce ? 1 : 0;
-  p.vec_io = ((reinterpret_cast<uintptr_t>(in) | (reduce ? 0 : reinterpret_cast<uintptr_t>(out))) & 15) == 0;
+  p.vec_io = ((reinterpret_cast<uintptr_t>(in) | (reduce ? 0 : reinterpret_cast<uintptr_t>(out))) & 3) == 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.epoch = epoch;
   p.ticket_base = ticket_base;

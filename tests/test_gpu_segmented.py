@@ -181,3 +181,22 @@ def test_segmented_plan_first_capture_allocates(gpu, q):
         assert z[-1].item() == 1000.0
         plan.close()
     cache.close()
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_segmented_views_one_element_off(gpu, q, scratch, dtype):
+    # input and output views one element into their buffers (not 16-byte
+    # aligned): 16-byte accesses at element alignment
+    torch, _ = gpu
+    off = lognormal_offsets(3000, seed=11, max_len=1 << 16)
+    n = int(off[-1])
+    x = rand(dtype, n + 1, seed=12, lo=-1000, hi=1000)
+    xd = dev(torch, x)[1:]
+    od = dev(torch, off)
+    yb = torch.zeros(n + 1, dtype=xd.dtype, device="cuda")
+    acpp.segmented_inclusive_scan(q, scratch, xd, yb[1:], od, acpp.PLUS)
+    out = torch.zeros(len(off) - 1, dtype=xd.dtype, device="cuda")
+    acpp.segmented_reduce(q, scratch, xd, od, out, dtype(0), acpp.PLUS)
+    q.wait()
+    assert np.array_equal(host(yb)[1:], O.segmented_inclusive_scan(x[1:], off))
+    assert np.array_equal(host(out), O.segmented_reduce(x[1:], off, dtype(0), acpp.PLUS))
