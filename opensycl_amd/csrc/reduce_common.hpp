@@ -162,6 +162,18 @@ __device__ __forceinline__ u32x4 load16(const T *base, uint64_t chunk) {
     return reinterpret_cast<const u32x4 *>(base)[chunk];
 }
 
+// The same load at 4-byte alignment (global_load_dwordx4 needs dword
+// alignment only): the second operand of a dot product of 4- or 8-byte
+// elements, whose misalignment may differ from the first operand's.
+typedef unsigned int u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+template <bool NT, class T>
+__device__ __forceinline__ u32x4 load16_a4(const T *base, uint64_t chunk) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(reinterpret_cast<const u32x4_a4 *>(base) + chunk);
+  else
+    return reinterpret_cast<const u32x4_a4 *>(base)[chunk];
+}
+
 template <class T, class W, int VEC>
 __device__ __forceinline__ void unpack(u32x4 q, W (&out)[VEC]) {
   using S = typename elem_traits<T>::storage;
@@ -197,7 +209,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
       qa[u] = load16<NT>(va, c + u * stride);
-      if constexpr (DOT) qb[u] = load16<NT>(vb, c + u * stride);
+      if constexpr (DOT) qb[u] = load16_a4<NT>(vb, c + u * stride);
     }
     if constexpr (XF && VEC > 4) {
       // Narrow elements (8 or 16 per load): one uniform switch per row.  A
@@ -245,7 +257,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     if constexpr (XF) unary_detail::apply_row<T>(p.unary, xa);
     if constexpr (DOT) {
       W xb[VEC];
-      unpack<T, W, VEC>(load16<NT>(vb, c), xb);
+      unpack<T, W, VEC>(load16_a4<NT>(vb, c), xb);
 #pragma unroll
       for (int k = 0; k < VEC; ++k) acc[0] = M::insert2(acc[0], xa[k], xb[k]);
     } else {

@@ -60,12 +60,18 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": items_per_thread must be 4 with an element transform");
 
-  // 16-byte aligned region; a dot product needs both operands equally
-  // misaligned, otherwise everything goes through the scalar loop.
+  // Scalar head, then the 16-byte aligned region of a, then a scalar tail.
   const uintptr_t abytes = reinterpret_cast<uintptr_t>(a);
   uint64_t head = ((16 - (abytes & 15)) & 15) / sizeof(typename elem_traits<T>::storage);
   bool vec_ok = (abytes % sizeof(typename elem_traits<T>::storage)) == 0;
-  if (DOT) vec_ok = vec_ok && ((reinterpret_cast<uintptr_t>(b) & 15) == (abytes & 15));
+  // The head aligns a; b is read with dword-aligned 16-byte loads, so for
+  // 4- and 8-byte elements it only has to be element-aligned (2-byte
+  // elements need b misaligned exactly like a).
+  if (DOT) {
+    constexpr size_t es = sizeof(typename elem_traits<T>::storage);
+    const uintptr_t bbytes = reinterpret_cast<uintptr_t>(b);
+    vec_ok = vec_ok && (es >= 4 ? (bbytes % es) == 0 : (bbytes & 15) == (abytes & 15));
+  }
   if (!vec_ok || head > n) head = n;
   const uint64_t nvec = (n - head) / VEC;
   const uint64_t tail_start = head + nvec * VEC;
