@@ -61,41 +61,53 @@ __device__ __forceinline__ void dispatch(int code, F &&f) {
   }
 }
 
-template <class T, class Op>
+// Fold of one input: `unroll` independent accumulators over 16-byte loads
+// (`unroll` in flight), non-temporal when NT, then the scalar tail.  (With one
+// accumulator and a bounds check per load, one fused int32 reduction of 2^28
+// elements took 175 us against 154 us for the plain reduce.)
+template <class T, class Op, bool NT>
 __device__ typename elem_traits<T>::work stream_fold(const T *in, uint64_t n) {
   using W = typename elem_traits<T>::work;
   constexpr int VEC = 16 / sizeof(T);
   const uint64_t gid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  W acc = Op::identity();
+  W acc[unroll];
+#pragma unroll
+  for (int u = 0; u < unroll; ++u) acc[u] = Op::identity();
   uint64_t scalar_from = 0;
   if ((reinterpret_cast<uintptr_t>(in) & 15) == 0) {
     const uint64_t nvec = n / VEC;
     const u32x4 *src = reinterpret_cast<const u32x4 *>(in);
-    for (uint64_t i = gid; i < nvec; i += stride * unroll) {
+    auto fold = [&](u32x4 q, W &a) {
+      T t[VEC];
+      __builtin_memcpy(t, &q, 16);
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) a = Op::apply(a, elem_traits<T>::to_work(t[k]));
+    };
+    uint64_t i = gid;
+    for (; i + (unroll - 1) * stride < nvec; i += unroll * stride) {
       u32x4 q[unroll];
 #pragma unroll
       for (int u = 0; u < unroll; ++u) {
-        const uint64_t j = i + u * stride;
-        if (j < nvec) q[u] = src[j];
+        if constexpr (NT)
+          q[u] = __builtin_nontemporal_load(src + i + u * stride);
+        else
+          q[u] = src[i + u * stride];
       }
 #pragma unroll
-      for (int u = 0; u < unroll; ++u) {
-        if (i + u * stride < nvec) {
-          T t[VEC];
-          __builtin_memcpy(t, &q[u], 16);
-#pragma unroll
-          for (int k = 0; k < VEC; ++k) acc = Op::apply(acc, elem_traits<T>::to_work(t[k]));
-        }
-      }
+      for (int u = 0; u < unroll; ++u) fold(q[u], acc[u]);
     }
+    for (; i < nvec; i += stride) fold(src[i], acc[0]);
     scalar_from = nvec * VEC;
   }
-  for (uint64_t i = scalar_from + gid; i < n; i += stride) acc = Op::apply(acc, elem_traits<T>::to_work(in[i]));
-  return acc;
+  for (uint64_t i = scalar_from + gid; i < n; i += stride) acc[0] = Op::apply(acc[0], elem_traits<T>::to_work(in[i]));
+  W v = acc[0];
+#pragma unroll
+  for (int u = 1; u < unroll; ++u) v = Op::apply(v, acc[u]);
+  return v;
 }
 
-template <class T>
+template <class T, bool NT>
 __global__ __launch_bounds__(1024) void multi_kernel(params<T> p) {
   using W = typename elem_traits<T>::work;
   __shared__ W s_waves[max_waves];
@@ -104,7 +116,7 @@ __global__ __launch_bounds__(1024) void multi_kernel(params<T> p) {
     __syncthreads();  // s_waves of the previous input has been read
     dispatch<W>(p.op[k], [&](auto opv) {
       using Op = decltype(opv);
-      W v = stream_fold<T, Op>(p.in[k], p.n);
+      W v = stream_fold<T, Op, NT>(p.in[k], p.n);
       v = block_reduce<plain_monoid<W, Op>>(v, s_waves);
       if (threadIdx.x == 0) store_partial(p.partials + static_cast<uint64_t>(k) * gridDim.x + blockIdx.x, v);
     });
@@ -188,7 +200,7 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, uint32_t count, 
   if (!valid_work_group_size(wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad work_group_size");
   int resident = 1;
-  st = occupancy(multi_kernel<T>, static_cast<int>(wg), &resident);
+  st = occupancy(multi_kernel<T, false>, static_cast<int>(wg), &resident);
   if (st != ACPP_SUCCESS) return st;
   const uint32_t gpc = (cfg && cfg->groups_per_cu) ? cfg->groups_per_cu : 1;
   const uint64_t cap = static_cast<uint64_t>(std::min<uint32_t>(gpc, static_cast<uint32_t>(resident))) * cus;
@@ -204,7 +216,13 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, uint32_t count, 
   if (st != ACPP_SUCCESS) return st;
   p.partials = static_cast<W *>(partials);
   p.sync = static_cast<sync_header *>(sync);
-  hipLaunchKernelGGL(multi_kernel<T>, dim3(grid), dim3(wg), 0, stream, p);
+  // Non-temporal loads once the inputs are larger than the 256 MiB
+  // last-level cache, as for the plain reductions.
+  const bool nt = static_cast<uint64_t>(count) * n * sizeof(T) >= (256ull << 20);
+  if (nt)
+    hipLaunchKernelGGL((multi_kernel<T, true>), dim3(grid), dim3(wg), 0, stream, p);
+  else
+    hipLaunchKernelGGL((multi_kernel<T, false>), dim3(grid), dim3(wg), 0, stream, p);
   st = check_launch(name);
   return st == ACPP_SUCCESS ? clear_error() : st;
 }
