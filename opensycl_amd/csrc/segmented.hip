@@ -807,7 +807,16 @@ __global__ void segmented_fixup_kernel(params<T> p, const typename elem_traits<T
     const uint64_t tl = (e - 1) / tile_elems;
     if (tb == tl || tl >= p.num_tiles) continue;
     W acc = tail_part[tb];
-    for (uint64_t t = tb + 1; t <= tl; ++t) acc = Op::apply(acc, head_part[t]);
+    uint64_t t = tb + 1;
+    // Eight partials in flight per step, folded in tile order.
+    for (; t + 8 <= tl + 1; t += 8) {
+      W h[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) h[u] = head_part[t + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = Op::apply(acc, h[u]);
+    }
+    for (; t <= tl; ++t) acc = Op::apply(acc, head_part[t]);
     p.out[s] = static_cast<T>(Op::apply(p.init, acc));
   }
 }
