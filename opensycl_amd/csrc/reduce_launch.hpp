@@ -51,7 +51,11 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   const uint32_t unroll = (cfg && cfg->items_per_thread) ? cfg->items_per_thread : (tuned && DOT ? (comp_dot ? 2 : 8) : 4);
   const uint32_t groups_per_cu = (cfg && cfg->groups_per_cu) ? cfg->groups_per_cu : 1;
   const uint64_t stream_bytes = n * sizeof(typename elem_traits<T>::storage) * (DOT ? 2 : 1);
-  const bool nt = cfg_given ? (cfg->flags & ACPP_VARIANT_NONTEMPORAL) != 0 : (stream_bytes >= (256ull << 20));
+  // 1-byte sums are bound by per-element arithmetic, not by the cache:
+  // non-temporal loads tied at 2^30 int8 and lost at 2^28 (52.7 us against
+  // 42.8 us), so they keep ordinary loads.
+  const bool nt = cfg_given ? (cfg->flags & ACPP_VARIANT_NONTEMPORAL) != 0
+                            : (!narrow8 && stream_bytes >= (256ull << 20));
   if (unroll != 2 && unroll != 4 && unroll != 8 && (unroll != 16 || DOT))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": items_per_thread must be 2, 4, 8 or 16 (16 only "
