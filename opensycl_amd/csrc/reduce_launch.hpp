@@ -133,11 +133,10 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     if (unroll == 8) return run(reduce_kernel<T, M, Op, OutT, 8, DOT>);
     if (unroll == 16) return run(reduce_kernel<T, M, Op, OutT, 16, DOT>);
   }
-  // Other operators: 4 loads in flight, non-temporal above 256 MiB as for
-  // sums (2^28 int32 max: 170 us with ordinary loads).
-  if constexpr (!XF) {
-    if (nt) return run(reduce_kernel<T, M, Op, OutT, 4, DOT, true, false>);
-  }
+  // Other operators and the transform forms: 4 loads in flight,
+  // non-temporal above 256 MiB as for sums (2^28 int32 max: 170 us with
+  // ordinary loads).
+  if (nt) return run(reduce_kernel<T, M, Op, OutT, 4, DOT, true, XF>);
   return run(reduce_kernel<T, M, Op, OutT, 4, DOT, false, XF>);
 }
 
