@@ -32,9 +32,9 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   // (int32 max: 213 us against 170 us; int32 squares through the transform
   // form: 211 us against 170 us).  1-byte elements take 1024 work-items: 16
   // elements per load make the per-element arithmetic the limit, and 512
-  // work-items left 2^30 int8 at 301 us against 190 us.  Sums (the tuned
-  // path) stream with non-temporal loads once the input is larger than the
-  // 256 MiB last-level cache; that choice is the library's only when the
+  // work-items left 2^30 int8 at 301 us against 190 us.  Inputs larger than
+  // the 256 MiB last-level cache stream with non-temporal loads (except
+  // 1-byte elements, below); that choice is the library's only when the
   // whole config is left at zero.  The compensated dot products keep 2 loads
   // in flight: their per-load Dot2 state makes 8 loads need 128 VGPRs and
   // spill, and 2 loads (45 VGPRs) measured fastest (2 x 2^28 fp32: 318 us
