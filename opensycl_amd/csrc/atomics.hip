@@ -69,13 +69,19 @@ __device__ __forceinline__ uint64_t scalar_index(const geom &g, uint64_t s) {
 // --------------------------------------------------------------------------
 // Histogram
 // --------------------------------------------------------------------------
+// copies_log2 (32-bit counters only): the group keeps 2^copies_log2 copies
+// of the histogram, word (bin << copies_log2) + copy, and work-item t counts
+// into copy t mod 2^copies_log2.  Lanes that meet the same hot bin then hit
+// adjacent words in different banks instead of one address.
 template <bool PACKED, bool GLOBAL, int UNROLL>
 __global__ __launch_bounds__(1024) void histogram_kernel(const uint32_t *__restrict__ keys,
                                                          geom gm, uint32_t num_bins,
-                                                         uint32_t *partials, uint32_t *bins) {
+                                                         uint32_t *partials, uint32_t *bins,
+                                                         uint32_t copies_log2) {
   extern __shared__ uint32_t lds[];
+  const uint32_t my_copy = threadIdx.x & ((1u << copies_log2) - 1u);
   if constexpr (!GLOBAL) {
-    const uint32_t words = PACKED ? (num_bins + 1) / 2 : num_bins;
+    const uint32_t words = PACKED ? (num_bins + 1) / 2 : (num_bins << copies_log2);
     for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) lds[i] = 0;
     __syncthreads();
   }
@@ -93,7 +99,7 @@ __global__ __launch_bounds__(1024) void histogram_kernel(const uint32_t *__restr
         __hip_atomic_fetch_add(&bins[k], 0x8000u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     } else {
-      atomicAdd(&lds[k], 1u);
+      atomicAdd(&lds[(k << copies_log2) + my_copy], 1u);
     }
   };
 
@@ -128,10 +134,12 @@ __global__ __launch_bounds__(1024) void histogram_kernel(const uint32_t *__restr
     uint32_t *mine = partials + static_cast<uint64_t>(blockIdx.x) * num_bins;
     for (uint32_t b = threadIdx.x; b < num_bins; b += blockDim.x) {
       uint32_t v;
-      if constexpr (PACKED)
+      if constexpr (PACKED) {
         v = (lds[b >> 1] >> ((b & 1u) * 16u)) & 0xffffu;
-      else
-        v = lds[b];
+      } else {
+        v = 0;
+        for (uint32_t c = 0; c < (1u << copies_log2); ++c) v += lds[(b << copies_log2) + c];
+      }
       mine[b] = v;
     }
   }
@@ -360,13 +368,19 @@ acpp_status histogram(acpp_allocation_group *g, hipStream_t stream, const uint32
     if (cfg && cfg->groups_per_cu)
       groups_cap = std::min<uint64_t>(groups_cap, cfg->groups_per_cu * static_cast<uint64_t>(cus));
     const uint32_t grid = pick_grid(gm.nvec + gm.nscalar, wg * 4, groups_cap);
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(wg), 0, stream, keys, gm, num_bins, nullptr, bins);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(wg), 0, stream, keys, gm, num_bins, nullptr, bins, 0u);
     st = check_launch(name);
     return st == ACPP_SUCCESS ? clear_error() : st;
   }
 
   const bool packed = !fits32;
-  const size_t lds = packed ? static_cast<size_t>((num_bins + 1) / 2) * 4 : static_cast<size_t>(num_bins) * 4;
+  // Up to 8 copies of 32-bit counters within the LDS budget.
+  uint32_t copies_log2 = 0;
+  while (!packed && copies_log2 < 3 &&
+         (static_cast<uint64_t>(num_bins) << (copies_log2 + 1)) * 4 <= lds_bytes_budget)
+    ++copies_log2;
+  const size_t lds = packed ? static_cast<size_t>((num_bins + 1) / 2) * 4
+                            : (static_cast<size_t>(num_bins) << copies_log2) * 4;
   auto kernel = packed ? histogram_kernel<true, false, 4> : histogram_kernel<false, false, 4>;
   int resident = 1;
   ACPP_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kernel, static_cast<int>(wg), lds));
@@ -387,7 +401,7 @@ acpp_status histogram(acpp_allocation_group *g, hipStream_t stream, const uint32
   st = group_obtain(g, static_cast<size_t>(grid) * num_bins * sizeof(uint32_t), &partials);
   if (st != ACPP_SUCCESS) return st;
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(wg), lds, stream, keys, gm, num_bins,
-                     static_cast<uint32_t *>(partials), bins);
+                     static_cast<uint32_t *>(partials), bins, copies_log2);
   st = check_launch(name);
   if (st != ACPP_SUCCESS) return st;
   const uint32_t chunks = std::min<uint32_t>(grid, 32u);
