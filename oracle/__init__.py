@@ -74,6 +74,11 @@ def max_threads() -> int:
     return load().oracle_max_threads()
 
 
+def num_procs() -> int:
+    """omp_get_num_procs(): the host device's max_compute_units in the reference."""
+    return load().oracle_num_procs()
+
+
 def identity(dtype, op: int):
     t = _NP_TAG[np.dtype(dtype)]
     v = _CT[t]()
@@ -274,17 +279,6 @@ def fetch_max_strided(values: np.ndarray, num_slots: int, slots: np.ndarray | No
     _ok(load().oracle_fetch_max_strided_i32(c_void_p(_p(values)), c_uint64(values.size),
                                             c_void_p(_p(s)), c_uint32(num_slots), c_int(threads)))
     return s
-
-
-def fetch_op_indexed(rmw_op: int, values: np.ndarray, idx: np.ndarray,
-                     target: np.ndarray) -> np.ndarray:
-    values = np.ascontiguousarray(values, dtype=np.int32)
-    idx = np.ascontiguousarray(idx, dtype=np.uint32)
-    t = np.ascontiguousarray(target, dtype=np.int32).copy()
-    _ok(load().oracle_fetch_op_indexed_i32(c_int(rmw_op), c_void_p(_p(values)),
-                                           c_void_p(_p(idx)), c_uint64(values.size),
-                                           c_void_p(_p(t)), c_uint32(t.size)))
-    return t
 
 
 # copy_if predicate codes (== ACPP_PRED_*)

@@ -19,6 +19,7 @@ static int g_fault = 0;
 void oracle_set_fault(int fault) { g_fault = fault; }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+int oracle_num_procs(void) { return omp_get_num_procs(); }
 
 enum { OP_PLUS = 0, OP_MUL, OP_MIN, OP_MAX, OP_BAND, OP_BOR, OP_BXOR, OP_LAND, OP_LOR };
 /* rmw operation numbering of the atomic builtin ABI
@@ -216,9 +217,17 @@ static inline double refid_f64(int op) {
      - *out = op(init, result) (reduction_descriptor.hpp:36-38).                                    \
      For integers this equals oracle_reduce_##t; for floating point it is the                       \
      reference's evaluation order, including its padding identities. */                            \
+  /* Faults 10 (adjacent pairs combined first) and 11 (operands swapped)                           \
+     restate the tree wrongly; the crafted-input tests must catch both. */                          \
   static void tree128_##t(int op, T *l) {                                                           \
+    if (g_fault == 10) {                                                                            \
+      for (int w = 1; w < 128; w *= 2)                                                              \
+        for (int i = 0; i + w < 128; i += 2 * w) l[i] = ap_##t(op, l[i], l[i + w]);                 \
+      return;                                                                                       \
+    }                                                                                               \
     for (int s = 64; s > 0; s /= 2)                                                                 \
-      for (int i = 0; i < s; ++i) l[i] = ap_##t(op, l[i], l[i + s]);                                \
+      for (int i = 0; i < s; ++i)                                                                   \
+        l[i] = (g_fault == 11) ? ap_##t(op, l[i + s], l[i]) : ap_##t(op, l[i], l[i + s]);           \
   }                                                                                                 \
   static void omp_stage_reduce_##t(int op, const T *in, uint64_t items, T *partials,                \
                                    uint64_t groups, int num_threads) {                              \
@@ -239,22 +248,32 @@ static inline double refid_f64(int op) {
                                 int num_threads) {                                                  \
     if (n == 0) return 0; /* numeric.hpp:165-169 */                                                 \
     if (num_threads <= 0) num_threads = omp_get_max_threads();                                      \
+    const int red_op = b ? OP_PLUS : op;                                                            \
+    if (g_fault == 9) { /* planted: a serial left fold instead of the reference's order */          \
+      T acc = init;                                                                                 \
+      for (uint64_t i = 0; i < n; ++i)                                                              \
+        acc = ap_##t(red_op, acc, b ? ap_##t(OP_MUL, a[i], b[i]) : a[i]);                           \
+      *out = acc;                                                                                   \
+      return 0;                                                                                     \
+    }                                                                                               \
+    /* Fault 8 plants 16 elements per work-item in place of 8. */                                   \
+    const uint64_t per_item = (g_fault == 8) ? 16 : 8;                                              \
     const uint64_t default_groups = (n + 127) / 128;                                                \
-    const uint64_t groups = (default_groups + 7) / 8;                                               \
+    const uint64_t groups = (default_groups + per_item - 1) / per_item;                             \
     T *stage = (T *)malloc(sizeof(T) * groups);                                                     \
     if (!stage) return -1;                                                                          \
-    const int red_op = b ? OP_PLUS : op;                                                            \
     _Pragma("omp parallel for num_threads(num_threads) schedule(static)")                          \
     for (uint64_t g = 0; g < groups; ++g) {                                                         \
       T l[128];                                                                                     \
       for (int lid = 0; lid < 128; ++lid) {                                                         \
         const uint64_t gid = g * 128 + lid;                                                         \
         T acc = refid_##t(red_op);                                                                  \
-        for (int i = 0; i < 8; ++i) {                                                               \
-          const uint64_t pos = 8 * gid + i;                                                         \
+        for (uint64_t i = 0; i < per_item; ++i) {                                                   \
+          const uint64_t pos = per_item * gid + i;                                                  \
           if (pos < n) {                                                                            \
             T v = b ? ap_##t(OP_MUL, a[pos], b[pos]) : a[pos];                                      \
-            acc = ap_##t(red_op, acc, v);                                                           \
+            /* fault 13 plants swapped operands in the work-item fold */                            \
+            acc = (g_fault == 13) ? ap_##t(red_op, v, acc) : ap_##t(red_op, acc, v);                \
           }                                                                                         \
         }                                                                                           \
         l[lid] = acc;                                                                               \
@@ -274,7 +293,8 @@ static inline double refid_f64(int op) {
       memcpy(stage, next, sizeof(T) * ng);                                                          \
       items = ng;                                                                                   \
     }                                                                                               \
-    *out = ap_##t(red_op, init, stage[0]);                                                          \
+    /* fault 14 plants op(result, init) in place of op(init, result) */                             \
+    *out = (g_fault == 14) ? ap_##t(red_op, stage[0], init) : ap_##t(red_op, init, stage[0]);       \
     free(next);                                                                                     \
     free(stage);                                                                                    \
     return 0;                                                                                       \
@@ -302,7 +322,8 @@ static inline double refid_f64(int op) {
     if (!valid_op_##t(op)) return -1;                                                               \
     if (n == 0) return 0;                                                                           \
     if (num_threads <= 0) num_threads = omp_get_max_threads();                                      \
-    const uint64_t per_group = 1024 * 16;                                                           \
+    /* fault 12 plants 1024 x 8 tiles in place of 1024 x 16 */                                      \
+    const uint64_t per_group = (g_fault == 12) ? 1024 * 8 : 1024 * 16;                              \
     const uint64_t groups = (n + per_group - 1) / per_group;                                        \
     T *aggregate = (T *)malloc(sizeof(T) * groups);                                                 \
     T *prefix = (T *)malloc(sizeof(T) * groups);                                                    \
@@ -572,18 +593,6 @@ int oracle_fetch_max_strided_i32(const int32_t *values, uint64_t n, int32_t *slo
       if (v > slots[s]) slots[s] = v;
     }
   free(priv);
-  return 0;
-}
-
-int oracle_fetch_op_indexed_i32(int op, const int32_t *values, const uint32_t *idx, uint64_t n,
-                                int32_t *target, uint32_t num_targets) {
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint32_t k = idx[i];
-    if (k >= num_targets) continue;
-    int32_t v;
-    if (oracle_atomic_fold_i32(op, &values[i], 1, target[k], &v) != 0) return -1;
-    target[k] = v;
-  }
   return 0;
 }
 

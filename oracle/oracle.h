@@ -79,10 +79,6 @@ int oracle_histogram_u32(const uint32_t *keys, uint64_t n, uint32_t *bins, uint3
 /* slots[i % num_slots] = max(slots[i % num_slots], values[i]). */
 int oracle_fetch_max_strided_i32(const int32_t *values, uint64_t n, int32_t *slots,
                                  uint32_t num_slots, int num_threads);
-/* Generic indexed fetch op: target[idx[i]] = op(target[idx[i]], values[i]). */
-int oracle_fetch_op_indexed_i32(int op, const int32_t *values, const uint32_t *idx, uint64_t n,
-                                int32_t *target, uint32_t num_targets);
-
 /* copy_if: out[0..count) = the x[i] (in order) for which the predicate
    holds (codes: 0 x != 0, 1 ==, 2 !=, 3 <, 4 <=, 5 >, 6 >=, 7 (x & mask) ==
    operand, 8 stencil[i] != 0). */
@@ -95,9 +91,18 @@ int oracle_copy_if_f64(const double *x, uint64_t n, const uint8_t *stencil, int 
 
 /* Number of OpenMP threads a parallel region would use by default. */
 int oracle_max_threads(void);
+/* omp_get_num_procs(): the processors available to this process, the value
+   the reference reports as the host device's max_compute_units
+   (src/runtime/omp/omp_hardware_manager.cpp:124-128). */
+int oracle_num_procs(void);
 
 /* Deliberate-mistake switch for the tests that show the oracle's pinning
-   checks catch plausible errors.  0 = faithful (default). */
+   checks catch plausible errors.  0 = faithful (default).  1 dropped last
+   element, 2 inclusive for exclusive, 3 ignored init, 4 short group, 5 lost
+   histogram key, 6 short copy_if count, 7 abs(x) = x; in the OpenMP-order
+   restatements: 8 sixteen elements per work-item, 9 serial left fold, 10
+   adjacent-pairs tree, 11 swapped tree operands, 12 1024 x 8 scan tiles, 13
+   swapped work-item fold operands, 14 op(result, init). */
 void oracle_set_fault(int fault);
 
 /* Element transforms (transform codes 0..7 of acpp_unary_op); -1 for an

@@ -14,13 +14,21 @@ lost histogram key.
 
 Pinning status per oracle function:
   reduce, reduction, dot, inclusive/exclusive scan, group reduce/scans,
-  atomic fold, segmented scan/reduce, histogram, fetch_max, FP64 sums: pinned
-  (golden vectors + numpy / fsum / closed forms).
+  atomic fold, segmented scan/reduce, histogram, fetch_max, FP64 sums and the
+  FP64 absolute sum: pinned (golden vectors + numpy / fsum / closed forms).
   *_omp_order for integers: pinned (equal to numpy for every op).
-  *_omp_order for floating point: parity unpinned -- the bit pattern of the
-  reference's OpenMP evaluation order can only be pinned by running the
-  reference, which cannot be built here (no Boost.Fiber, no LLVM dev
-  packages); it is checked only for closeness to the FP64 oracle.
+  *_omp_order for floating point: pinned by crafted inputs whose result under
+  the reference's evaluation order has a closed form (section "OpenMP
+  evaluation order" below): values at 2^24 that absorb later ones expose how
+  many elements a work-item folds serially (8), the shape of the 128-wide
+  tree (stride halving), the stages over partials and the scan's tile length
+  (1024 x 16); signed zeros under sycl::minimum / maximum, which return their
+  second operand when both are zeros, expose every operand order (work-item
+  fold op(acc, x), tree op(l[i], l[i+s]), final op(init, r)).  Planted
+  faults 8-14 restate each of these wrongly and must fail.  The one part left
+  open is the look-back's grouping of several aggregates in a multi-threaded
+  scan, which the reference leaves to thread timing; the crafted inputs keep
+  every such aggregate exact, so their result does not depend on it.
 """
 import json
 import math
@@ -370,11 +378,160 @@ def test_omp_order_int_equals_serial(dt):
 
 
 def test_omp_order_fp_close_to_fp64():
-    # parity unpinned (see module docstring): closeness only.
     x = np.random.default_rng(9).standard_normal(1 << 20).astype(np.float32)
     exact = O.sum_fp64(x)
     r = float(O.reduce_omp_order(x, np.float32(0)))
     assert abs(r - exact) <= 1e-5 * O.abs_sum_fp64(x)
+
+
+# Crafted inputs.  In float32 every integer up to 2^24 is exact and the
+# spacing above 2^24 is 2, so 2^24 + 1 rounds back to 2^24 (tie to even):
+# ones folded serially into 2^24 are lost, ones summed among themselves first
+# are kept.  fl32() is numpy's round-to-nearest-even conversion from float64.
+T24 = float(1 << 24)
+
+
+def fl32(v):
+    return np.float32(np.float64(v))
+
+
+def check_omp_reduce_chunking():
+    # 1024 elements = one 128-wide group of 8-element work-items.  Work-item 0
+    # folds 2^24 and its 7 ones serially (the ones are lost); the other 127
+    # items hold 8 ones each, and the tree adds 2^24 + 8 * 127 exactly.
+    # 16 per item would give 2^24 + 1008, a serial fold 2^24.
+    x = np.ones(1024, dtype=np.float32)
+    x[0] = T24
+    ok = O.reduce_omp_order(x, np.float32(0), O.PLUS, 2) == fl32(T24 + 1016)
+    a = np.ones(1024, dtype=np.float32)
+    a[0] = 4096.0
+    ok &= O.dot_omp_order(a, a.copy(), np.float32(0), 2) == fl32(T24 + 1016)
+    return bool(ok)
+
+
+def check_omp_reduce_tree_shape():
+    # One group; work-items 1 and 65 hold a one each, item 0 holds 2^24.  The
+    # stride-halving tree first joins items i and i + 64, so the two ones meet
+    # (2) before reaching 2^24: 2^24 + 2.  Adjacent pairs would add each one
+    # to 2^24 on its own and lose both.
+    x = np.zeros(1024, dtype=np.float32)
+    x[0], x[8 * 1], x[8 * 65] = T24, 1.0, 1.0
+    ok = O.reduce_omp_order(x, np.float32(0), O.PLUS, 2) == fl32(T24 + 2)
+    # The same pattern one stage up: 130 groups; the partials of groups 0, 1
+    # and 65 are 2^24, 1 and 1 and meet in the second stage's 128-wide tree.
+    y = np.zeros(1024 * 130, dtype=np.float32)
+    y[0], y[1024 * 1], y[1024 * 65] = T24, 1.0, 1.0
+    ok &= O.reduce_omp_order(y, np.float32(0), O.PLUS, 2) == fl32(T24 + 2)
+    return bool(ok)
+
+
+def check_omp_operand_order():
+    # sycl::minimum is x < y ? x : y: on two zeros it returns the second
+    # operand, and on a zero and the padding identity (the largest float) it
+    # returns the zero.  The reference combines every pair as op(earlier,
+    # later) -- op(acc, x) in a work-item, op(l[i], l[i + s]) in the tree,
+    # op(init, r) at the end -- so over zeros the result is the zero the
+    # tree's last combine takes as its second operand.  Following l[0] <-
+    # l[1] <- l[3] <- ... <- l[63] <- l[127] back to a leaf: with full groups
+    # that is the last element of work-item 127, i.e. the last input element
+    # (n = 1024, and n = 1024 * 128 whose 128 partials fill the second stage).
+    # With n = 1000 items 125..127 are padding, l[63] keeps its own value at
+    # the stride-64 step, and the result is the last element of item 63,
+    # x[511].  (sycl::maximum cannot show this: its padding identity,
+    # numeric_limits<float>::min(), beats every zero.)
+    ok = True
+    for n, decisive in ((1024, 1023), (1024 * 128, 1024 * 128 - 1), (1000, 511)):
+        for zero, other in ((-0.0, 0.0), (0.0, -0.0)):
+            x = np.full(n, other, dtype=np.float32)
+            x[decisive] = zero
+            r = O.reduce_omp_order(x, np.float32(other), O.MIN, 2)
+            ok &= bool(r == 0 and np.signbit(r) == np.signbit(np.float32(zero)))
+    # the quirk itself: maximum over zeros returns the padding value
+    r = O.reduce_omp_order(np.zeros(1024, dtype=np.float32), np.float32(-1.0), O.MAX, 2)
+    ok &= bool(r == np.finfo(np.float32).tiny)
+    return ok
+
+
+def check_omp_scan_tile_length():
+    # 2^24 then ones.  Tile 0 (16384 elements, scanned serially) absorbs its
+    # ones: y = 2^24.  Later tiles scan their own ones exactly and add the
+    # exact prefix of earlier tiles, so y[i] = fl32(2^24 + i - 16383).  With
+    # 8192-element tiles the rise would start at 8192.
+    n = 3 * 16384
+    x = np.ones(n, dtype=np.float32)
+    x[0] = T24
+    i = np.arange(n, dtype=np.float64)
+    want = (T24 + np.maximum(i - 16383, 0)).astype(np.float32)
+    ok = np.array_equal(O.scan_omp_order(x, False, None, O.PLUS, 3), want)
+    ex = O.scan_omp_order(x, True, np.float32(0), O.PLUS, 3)
+    want_ex = want.copy()
+    want_ex[0] = 0.0
+    ok &= np.array_equal(ex, want_ex)
+    return bool(ok)
+
+
+def test_omp_reduce_ones_above_2e24():
+    # 2^25 ones: every partial of the reference's order (8 per item, trees of
+    # 128) is an exact power of two, so the sum is exact; a serial fold stops
+    # at 2^24.
+    x = np.ones(1 << 25, dtype=np.float32)
+    assert O.reduce_omp_order(x, np.float32(0)) == np.float32(1 << 25)
+    O.set_fault(9)
+    assert O.reduce_omp_order(x, np.float32(0)) == np.float32(1 << 24)
+
+
+def test_omp_scan_ones_above_2e24():
+    # Every tile aggregate and prefix is exact, so y[i] = fl32(i + 1): the
+    # rounding of the exact count, ties to even, past 2^24.  A serial scan
+    # would stay at 2^24.
+    n = (1 << 24) + 3 * 16384 + 5
+    x = np.ones(n, dtype=np.float32)
+    want = (np.arange(n, dtype=np.float64) + 1).astype(np.float32)
+    assert np.array_equal(O.scan_omp_order(x, False, None, O.PLUS, 4), want)
+    assert want[(1 << 24) + 2] == np.float32(T24 + 4)
+
+
+def test_omp_dot_rounds_each_product():
+    # transform then reduce: the product is rounded to float32 before the
+    # add (numeric.hpp:204-223).  (-1)(1) + (1 + 2^-12)^2: the rounded product
+    # 1 + 2^-11 gives 2^-11; a fused multiply-add would keep the 2^-24.
+    a = np.array([-1.0, 1.0 + 2.0 ** -12], dtype=np.float32)
+    b = np.array([1.0, 1.0 + 2.0 ** -12], dtype=np.float32)
+    assert O.dot_omp_order(a, b, np.float32(0), 1) == np.float32(2.0 ** -11)
+    assert O.dot_fp64(a, b) == 2.0 ** -11 + 2.0 ** -24
+
+
+@pytest.mark.parametrize("check", [check_omp_reduce_chunking, check_omp_reduce_tree_shape,
+                                   check_omp_operand_order, check_omp_scan_tile_length])
+def test_omp_order_crafted(check):
+    assert check()
+
+
+@pytest.mark.parametrize("fault,check", [
+    (8, check_omp_reduce_chunking),
+    (9, check_omp_reduce_chunking),
+    (10, check_omp_reduce_tree_shape),
+    (11, check_omp_operand_order),
+    (12, check_omp_scan_tile_length),
+    (13, check_omp_operand_order),
+    (14, check_omp_operand_order),
+])
+def test_omp_order_planted_mistakes_are_caught(fault, check):
+    O.set_fault(fault)
+    try:
+        assert not check()
+    finally:
+        O.set_fault(0)
+
+
+def test_abs_sum_fp64_vs_fsum():
+    rng = np.random.default_rng(10)
+    x = rng.standard_normal(300007).astype(np.float32)
+    x[rng.random(x.size) < 0.01] *= np.float32(-1e4)
+    assert O.abs_sum_fp64(x) == math.fsum(np.abs(x.astype(np.float64)).tolist())
+    # closed form: sum of |i - 1000| over 0..2999 = 1000*1001/2 + 2000*1999/2
+    i = np.arange(3000, dtype=np.float32) - np.float32(1000)
+    assert O.abs_sum_fp64(i) == 1000 * 1001 / 2 + 1999 * 2000 / 2
 
 
 def test_omp_order_reference_max_identity_quirk():
