@@ -25,9 +25,11 @@
 //   - reduction / scan operators: acpp::plus ... acpp::logical_or and the
 //     std:: functors of the same operations (acpp/functional.hpp);
 //     floating-point types take plus, multiplies, minimum and maximum;
-//   - binary transform_reduce: std::plus / acpp::plus with std::multiplies /
-//     acpp::multiplies (the dot product, numeric.hpp:215-223) for int32,
-//     int64, uint32, uint64, float and double;
+//   - binary transform_reduce: reduce = plus, multiplies, minimum or maximum
+//     with a pair transform (multiplies -- with plus, the dot product,
+//     numeric.hpp:215-223 -- plus, minus, absolute_difference,
+//     squared_difference, minimum, maximum, bit_and/or/xor) for int32, int64,
+//     uint32, uint64, float and double;
 //   - unary transform_reduce and transform_inclusive_scan /
 //     transform_exclusive_scan (numeric.hpp:193-212, 299-327): any of the
 //     operators above with an element transform from acpp/functional.hpp
@@ -140,6 +142,32 @@ ACPP_FACADE_DOT(float, f32, float)
 ACPP_FACADE_DOT(double, f64, double)
 #undef ACPP_FACADE_DOT
 
+template <class T>
+struct binary_abi {
+  static constexpr bool supported = false;
+};
+#define ACPP_FACADE_BINARY(CT, S, AT)                                                              \
+  template <>                                                                                      \
+  struct binary_abi<CT> {                                                                          \
+    static constexpr bool supported = true;                                                        \
+    static acpp_status run(acpp_allocation_group *g, hipStream_t s, const CT *a, uint64_t n,      \
+                           const CT *b, CT *out, CT init, int op, int code) {                      \
+      return acpp_transform_reduce_binary_##S(g, s, reinterpret_cast<const AT *>(a), n,            \
+                                              reinterpret_cast<const AT *>(b),                     \
+                                              reinterpret_cast<AT *>(out), static_cast<AT>(init),  \
+                                              op, code, nullptr);                                  \
+    }                                                                                              \
+  };
+ACPP_FACADE_BINARY(int, i32, int32_t)
+ACPP_FACADE_BINARY(long, i64, int64_t)
+ACPP_FACADE_BINARY(long long, i64, int64_t)
+ACPP_FACADE_BINARY(unsigned int, u32, uint32_t)
+ACPP_FACADE_BINARY(unsigned long, u64, uint64_t)
+ACPP_FACADE_BINARY(unsigned long long, u64, uint64_t)
+ACPP_FACADE_BINARY(float, f32, float)
+ACPP_FACADE_BINARY(double, f64, double)
+#undef ACPP_FACADE_BINARY
+
 // Contiguous iterators: pointers, and the pointer-wrapping iterators of
 // contiguous standard containers (std::vector with any allocator).
 template <class It>
@@ -206,19 +234,31 @@ inline event finish(queue &q, acpp_status st) {
 
 // ------------------------------------------------------------------ reduce
 
+// Binary form (numeric.hpp:170-191): *out = reduce(init, transform(x[i], y[i])...).
+// reduce: plus, multiplies, minimum or maximum; transform: a pair transform
+// of acpp/functional.hpp (multiplies, plus, minus, absolute_difference,
+// squared_difference, minimum, maximum, bit_and, bit_or, bit_xor; std::
+// functors of the same operations).  plus with multiplies is the dot product.
 template <class ForwardIt1, class ForwardIt2, class T, class BinaryReductionOp,
           class BinaryTransformOp>
 event transform_reduce(queue &q, util::allocation_group &scratch, ForwardIt1 first1,
                        ForwardIt1 last1, ForwardIt2 first2, T *out, T init,
                        BinaryReductionOp reduce, BinaryTransformOp transform,
                        const std::vector<event> &deps = {}) {
-  static_assert(detail::dot_abi<T>::supported,
-                "acpp::algorithms::transform_reduce: dot product of int32/int64/uint32/uint64/"
-                "float/double");
-  static_assert(algorithm_op_of<std::decay_t<BinaryReductionOp>>::value == 0 &&
-                    algorithm_op_of<std::decay_t<BinaryTransformOp>>::value == 1,
-                "acpp::algorithms::transform_reduce: the library provides reduce = plus with "
-                "transform = multiplies (the dot product)");
+  static_assert(detail::binary_abi<T>::supported,
+                "acpp::algorithms::transform_reduce: the binary form takes int32/int64/uint32/"
+                "uint64/float/double");
+  constexpr int op = detail::op_code<BinaryReductionOp, T>();
+  static_assert(op <= 3,
+                "acpp::algorithms::transform_reduce: the binary form reduces with plus, multiplies, "
+                "minimum or maximum");
+  constexpr int code = binary_op_of<std::decay_t<BinaryTransformOp>>::value;
+  static_assert(code >= 0,
+                "acpp::algorithms::transform_reduce: the pair transform must be multiplies, plus, "
+                "minus, absolute_difference, squared_difference, minimum, maximum, bit_and, bit_or "
+                "or bit_xor");
+  static_assert(std::is_integral<T>::value || code <= 6,
+                "acpp::algorithms::transform_reduce: bitwise pair transforms need an integer type");
   static_assert(std::is_same<typename detail::contiguous<ForwardIt1>::value, T>::value &&
                     std::is_same<typename detail::contiguous<ForwardIt2>::value, T>::value,
                 "acpp::algorithms::transform_reduce: input value types must equal T");
@@ -227,9 +267,14 @@ event transform_reduce(queue &q, util::allocation_group &scratch, ForwardIt1 fir
   if (first1 == last1) return event{};
   const uint64_t n = static_cast<uint64_t>(std::distance(first1, last1));
   q.depends_on(deps);
-  return detail::finish(q, detail::dot_abi<T>::dot(scratch.native(), q.get_native(),
-                                                   detail::to_ptr(first1), n,
-                                                   detail::to_ptr(first2), out, init));
+  if constexpr (op == 0 && code == 0)
+    return detail::finish(q, detail::dot_abi<T>::dot(scratch.native(), q.get_native(),
+                                                     detail::to_ptr(first1), n,
+                                                     detail::to_ptr(first2), out, init));
+  else
+    return detail::finish(q, detail::binary_abi<T>::run(scratch.native(), q.get_native(),
+                                                        detail::to_ptr(first1), n,
+                                                        detail::to_ptr(first2), out, init, op, code));
 }
 
 template <class ForwardIt1, class ForwardIt2, class T>

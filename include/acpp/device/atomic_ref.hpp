@@ -425,6 +425,122 @@ ACPP_SSCP_ATOMIC_INT(u64, uint64_t)
 ACPP_SSCP_ATOMIC_COMMON(f32, float)
 ACPP_SSCP_ATOMIC_COMMON(f64, double)
 
+// ---------------------------------------------------------------------------
+// 8- and 16-bit entry points (sscp/builtins/atomic.hpp:16-311: store, load,
+// exchange, compare-exchange and fetch and/or/xor/add/sub/min/max for i8 and
+// i16; fetch add/sub/min/max for u8 and u16).  gfx950 has no 8- or 16-bit
+// atomic read-modify-write, so each is a compare-and-swap loop on the
+// aligned 32-bit word that holds the element: the other bytes of the word
+// are carried over unchanged, so neighbouring elements may be updated
+// concurrently.  Loads read the whole word.
+// ---------------------------------------------------------------------------
+namespace acpp {
+namespace detail {
+
+template <class T>
+struct subword {
+  static_assert(sizeof(T) == 1 || sizeof(T) == 2, "8- or 16-bit elements");
+  using U = std::make_unsigned_t<T>;
+  uint32_t *word;
+  uint32_t shift;
+  uint32_t mask;
+  __device__ explicit subword(T *p) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    word = reinterpret_cast<uint32_t *>(a & ~static_cast<uintptr_t>(3));
+    shift = static_cast<uint32_t>(a & 3) * 8;
+    mask = ((sizeof(T) == 1) ? 0xffu : 0xffffu) << shift;
+  }
+  __device__ T get(uint32_t w) const { return static_cast<T>(static_cast<U>((w & mask) >> shift)); }
+  __device__ uint32_t put(uint32_t w, T v) const {
+    return (w & ~mask) | ((static_cast<uint32_t>(static_cast<U>(v)) << shift) & mask);
+  }
+};
+
+// Replaces the element by f(old) atomically; returns old.
+template <class T, class F>
+__device__ __forceinline__ T subword_rmw(T *p, F f, memory_order o, memory_scope s) {
+  const subword<T> sw(p);
+  uint32_t w = atomic_load(sw.word, memory_order::relaxed, s);
+  for (;;) {
+    const T old = sw.get(w);
+    if (atomic_cas(sw.word, w, sw.put(w, f(old)), o, memory_order::relaxed, s, false)) return old;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T subword_load(T *p, memory_order o, memory_scope s) {
+  const subword<T> sw(p);
+  return sw.get(atomic_load(sw.word, o, s));
+}
+
+// Compare-exchange of one element.  Fails (and updates `expected`) only if
+// the element differs from `expected`; a change to the word's other bytes
+// makes it retry, so the weak form behaves like the strong one.
+template <class T>
+__device__ __forceinline__ bool subword_cas(T *p, T &expected, T desired, memory_order o, memory_scope s) {
+  const subword<T> sw(p);
+  uint32_t w = atomic_load(sw.word, memory_order::relaxed, s);
+  for (;;) {
+    const T cur = sw.get(w);
+    if (cur != expected) {
+      expected = cur;
+      return false;
+    }
+    if (atomic_cas(sw.word, w, sw.put(w, desired), o, memory_order::relaxed, s, false)) return true;
+  }
+}
+
+}  // namespace detail
+}  // namespace acpp
+
+#define ACPP_SSCP_SUBWORD_RMW(opname, tag, T, EXPR)                                               \
+  extern "C" __device__ __attribute__((always_inline)) inline T __acpp_sscp_atomic_##opname##_##tag( \
+      __acpp_sscp_address_space, __acpp_sscp_memory_order order, __acpp_sscp_memory_scope scope,  \
+      T *ptr, T x) {                                                                              \
+    return acpp::detail::subword_rmw(ptr, [x](T a) -> T { return EXPR; }, order, scope);        \
+  }
+
+#define ACPP_SSCP_SUBWORD_ARITH(tag, T)                                                          \
+  ACPP_SSCP_SUBWORD_RMW(fetch_add, tag, T, static_cast<T>(a + x))                                 \
+  ACPP_SSCP_SUBWORD_RMW(fetch_sub, tag, T, static_cast<T>(a - x))                                 \
+  ACPP_SSCP_SUBWORD_RMW(fetch_min, tag, T, (x < a) ? x : a)                                       \
+  ACPP_SSCP_SUBWORD_RMW(fetch_max, tag, T, (x > a) ? x : a)
+
+#define ACPP_SSCP_SUBWORD_FULL(tag, T)                                                           \
+  ACPP_SSCP_SUBWORD_ARITH(tag, T)                                                                 \
+  ACPP_SSCP_SUBWORD_RMW(fetch_and, tag, T, static_cast<T>(a & x))                                 \
+  ACPP_SSCP_SUBWORD_RMW(fetch_or, tag, T, static_cast<T>(a | x))                                  \
+  ACPP_SSCP_SUBWORD_RMW(fetch_xor, tag, T, static_cast<T>(a ^ x))                                 \
+  ACPP_SSCP_SUBWORD_RMW(exchange, tag, T, ((void)a, x))                                           \
+  extern "C" __device__ __attribute__((always_inline)) inline void __acpp_sscp_atomic_store_##tag( \
+      __acpp_sscp_address_space, __acpp_sscp_memory_order order, __acpp_sscp_memory_scope scope,  \
+      T *ptr, T x) {                                                                              \
+    (void)acpp::detail::subword_rmw(ptr, [x](T) -> T { return x; }, acpp::detail::store_order(order), scope); \
+  }                                                                                               \
+  extern "C" __device__ __attribute__((always_inline)) inline T __acpp_sscp_atomic_load_##tag(    \
+      __acpp_sscp_address_space, __acpp_sscp_memory_order order, __acpp_sscp_memory_scope scope,  \
+      T *ptr) {                                                                                   \
+    return acpp::detail::subword_load(ptr, order, scope);                                         \
+  }                                                                                               \
+  extern "C" __device__ __attribute__((always_inline)) inline bool __acpp_sscp_cmp_exch_strong_##tag( \
+      __acpp_sscp_address_space, __acpp_sscp_memory_order success, __acpp_sscp_memory_order,     \
+      __acpp_sscp_memory_scope scope, T *ptr, T *expected, T desired) {                           \
+    return acpp::detail::subword_cas(ptr, *expected, desired, success, scope);                    \
+  }                                                                                               \
+  extern "C" __device__ __attribute__((always_inline)) inline bool __acpp_sscp_cmp_exch_weak_##tag( \
+      __acpp_sscp_address_space, __acpp_sscp_memory_order success, __acpp_sscp_memory_order,     \
+      __acpp_sscp_memory_scope scope, T *ptr, T *expected, T desired) {                           \
+    return acpp::detail::subword_cas(ptr, *expected, desired, success, scope);                    \
+  }
+
+ACPP_SSCP_SUBWORD_FULL(i8, int8_t)
+ACPP_SSCP_SUBWORD_FULL(i16, int16_t)
+ACPP_SSCP_SUBWORD_ARITH(u8, uint8_t)
+ACPP_SSCP_SUBWORD_ARITH(u16, uint16_t)
+
+#undef ACPP_SSCP_SUBWORD_FULL
+#undef ACPP_SSCP_SUBWORD_ARITH
+#undef ACPP_SSCP_SUBWORD_RMW
 #undef ACPP_SSCP_ATOMIC_INT
 #undef ACPP_SSCP_ATOMIC_COMMON
 #undef ACPP_SSCP_ATOMIC_RMW

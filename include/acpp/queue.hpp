@@ -8,11 +8,15 @@
 //   acpp::event  -- shared handle to a hipEvent_t; a default-constructed
 //                   event is complete (the reference returns one for n == 0,
 //                   doc/algorithms.md:37);
-//   acpp::exception -- thrown for errors.  The reference reports errors
-//                   asynchronously through the queue's handler
-//                   (queue::wait_and_throw); the library behind this facade
-//                   detects errors when a call is enqueued, so the facade
-//                   throws at the call.
+//   acpp::exception -- thrown for errors.  Argument and launch errors are
+//                   detected when a call is enqueued, so the facade throws
+//                   at the call.  Errors that arise while kernels run (a
+//                   wait for another work-group that gave up, the library's
+//                   watchdog) are asynchronous, as in the reference
+//                   (queue::wait_and_throw / throw_asynchronous,
+//                   include/hipSYCL/sycl/queue.hpp:378-383): queue::wait()
+//                   and event::wait() throw them (code ACPP_ERROR_WATCHDOG)
+//                   once the work has completed.
 #pragma once
 
 #ifndef __HIP_PLATFORM_AMD__
@@ -51,10 +55,13 @@ inline void check_status(acpp_status s) {
 class event {
  public:
   event() = default;  // complete
-  explicit event(std::shared_ptr<hipEvent_t> e) : ev_(std::move(e)) {}
+  event(std::shared_ptr<hipEvent_t> e, int device) : ev_(std::move(e)), device_(device) {}
 
+  // Waits, then throws the device's asynchronous errors.
   void wait() const {
-    if (ev_) detail::check_hip(hipEventSynchronize(*ev_), "hipEventSynchronize");
+    if (!ev_) return;
+    detail::check_hip(hipEventSynchronize(*ev_), "hipEventSynchronize");
+    detail::check_status(acpp_check_asynchronous_errors(device_));
   }
   bool is_complete() const {
     if (!ev_) return true;
@@ -65,6 +72,7 @@ class event {
 
  private:
   std::shared_ptr<hipEvent_t> ev_;
+  int device_ = 0;
 };
 
 class queue {
@@ -89,7 +97,12 @@ class queue {
   int get_device() const { return device_; }
   hipStream_t get_native() const { return stream_; }
 
-  void wait() { detail::check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+  // Waits for everything enqueued so far, then throws the device's
+  // asynchronous errors (queue::wait_and_throw).
+  void wait() {
+    detail::check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    detail::check_status(acpp_check_asynchronous_errors(device_));
+  }
 
   // Make later work on this queue wait for `deps`.
   void depends_on(const std::vector<event> &deps) {
@@ -111,7 +124,7 @@ class queue {
       delete p;
     });
     detail::check_hip(hipEventRecord(*handle, stream_), "hipEventRecord");
-    return event(std::move(handle));
+    return event(std::move(handle), device_);
   }
 
  private:

@@ -45,6 +45,8 @@ typedef int acpp_status;
 #define ACPP_ERROR_UNSUPPORTED 2      /* op not defined for this element type */
 #define ACPP_ERROR_OUT_OF_MEMORY 3
 #define ACPP_ERROR_HIP 4              /* HIP runtime reported an error */
+#define ACPP_ERROR_WATCHDOG 5         /* a kernel gave up waiting: see      */
+                                      /* acpp_check_asynchronous_errors     */
 
 /* Message for the last failing call on this thread ("" if none). */
 const char *acpp_last_error_message(void);
@@ -87,6 +89,21 @@ typedef int32_t acpp_unary_op;
 #define ACPP_UNARY_IS_ZERO 6    /* x == 0 ? 1 : 0 */
 #define ACPP_UNARY_IS_NONZERO 7 /* x != 0 ? 1 : 0 */
 
+/* Pair transforms of the binary transform_reduce (numeric.hpp:170-191:
+   *out = reduce(init, transform(x[i], y[i]) ...)), in the element type;
+   integer results wrap.  The bitwise codes exist only for integer types. */
+typedef int32_t acpp_binary_op;
+#define ACPP_BINARY_MULTIPLIES 0 /* x * y (with plus: the dot product) */
+#define ACPP_BINARY_PLUS 1       /* x + y */
+#define ACPP_BINARY_MINUS 2      /* x - y */
+#define ACPP_BINARY_ABS_DIFF 3   /* |x - y| */
+#define ACPP_BINARY_SQ_DIFF 4    /* (x - y) * (x - y) */
+#define ACPP_BINARY_MIN 5        /* x < y ? x : y */
+#define ACPP_BINARY_MAX 6        /* x > y ? x : y */
+#define ACPP_BINARY_BIT_AND 7    /* x & y */
+#define ACPP_BINARY_BIT_OR 8     /* x | y */
+#define ACPP_BINARY_BIT_XOR 9    /* x ^ y */
+
 typedef void *acpp_stream_t; /* hipStream_t */
 
 /* ------------------------------------------------------------------------ */
@@ -117,6 +134,11 @@ typedef struct acpp_launch_config {
 #define ACPP_VARIANT_BLOCK_LOOKBACK 1u
 #define ACPP_VARIANT_NONTEMPORAL 2u
 #define ACPP_VARIANT_TILE_PER_GROUP 4u
+/* For tests of the error path only: every wait for another work-group
+   (look-backs, a reduction's wait for partials) gives up at once, as if
+   its watchdog had expired.  Results are wrong and the give-up is reported
+   as ACPP_ERROR_WATCHDOG; every kernel still ends normally. */
+#define ACPP_VARIANT_FORCE_GIVE_UP 8u
 
 /* ------------------------------------------------------------------------ */
 /* Scratch memory: allocation_cache / allocation_group                      */
@@ -150,12 +172,55 @@ acpp_status acpp_allocation_group_release(acpp_allocation_group *group);
 /* ~allocation_group -- allocation_cache.hpp:158-160 */
 acpp_status acpp_allocation_group_destroy(acpp_allocation_group *group);
 /* No counterpart in the reference (its look-back spins without bound).
-   Look-backs of the scans, copy_if and segmented scans give up after ~2.7 s
-   of waiting instead of hanging the device; such a give-up makes that
+   Look-backs of the scans, copy_if and segmented scans, and the wait of a
+   reduction's last work-group for the other groups' partials, give up
+   after ~2.7 s instead of hanging the device; such a give-up makes that
    call's results wrong.  *trips = the number of give-ups recorded in the
    scratch regions the group holds (since each region was last reset).
    Blocking; call once the group's work has completed.  Expected to be 0. */
 acpp_status acpp_allocation_group_watchdog_trips(acpp_allocation_group *group, uint64_t *trips);
+/* Asynchronous errors of `device`, the counterpart of the reference's
+   queue::throw_asynchronous / wait_and_throw (include/hipSYCL/sycl/
+   queue.hpp:378-383; src/runtime/error.cpp:77-90).  Every give-up above
+   also raises a per-device flag in pinned host memory; this call reads and
+   clears it without touching the device, and returns ACPP_ERROR_WATCHDOG
+   if it was raised (ACPP_SUCCESS otherwise).  Call it after synchronising
+   with the work in question (the C++ facade's queue/event wait, the stdpar
+   calls and the Python Queue.wait do).  acpp_allocation_group_release and
+   _destroy report a raised flag of the group's device the same way (after
+   releasing) but leave it raised: only this call clears it. */
+acpp_status acpp_check_asynchronous_errors(int device);
+
+/* ------------------------------------------------------------------------ */
+/* Standard-parallelism offload decision: whether a std::reduce /            */
+/* transform_reduce / *_scan(par_unseq) call runs on the GPU or on the host. */
+/* Replaces include/hipSYCL/std/stdpar/detail/offload.hpp:384-594 and the    */
+/* sampled-time interpolation of offload_heuristic_db.hpp:247-318: memory    */
+/* only the device can read forces the GPU, memory only the host can read    */
+/* forces the host; pinned host and managed memory are decided by the        */
+/* learned cost model (the MLP of the acpp_mlp_* kernels, trained on run     */
+/* times measured on MI355X), evaluated once per device into a table of      */
+/* crossover sizes.                                                          */
+/* ------------------------------------------------------------------------ */
+#define ACPP_STDPAR_OP_REDUCE 0
+#define ACPP_STDPAR_OP_TRANSFORM_REDUCE 1 /* unary transform */
+#define ACPP_STDPAR_OP_BINARY_TRANSFORM_REDUCE 2
+#define ACPP_STDPAR_OP_INCLUSIVE_SCAN 3   /* also transform_inclusive_scan */
+#define ACPP_STDPAR_OP_EXCLUSIVE_SCAN 4   /* also transform_exclusive_scan */
+#define ACPP_STDPAR_OFFLOAD_AUTO 0   /* the decision above (default) */
+#define ACPP_STDPAR_OFFLOAD_ALWAYS 1 /* every call that may offload does */
+#define ACPP_STDPAR_OFFLOAD_NEVER 2  /* every call that may stay on the host does */
+/* *offload = 1 (GPU) or 0 (host) for a call of operation `op` over n
+   elements of `elem_bytes` bytes touching the `num_pointers` given
+   pointers.  Invalid argument if the pointers mix device-only and
+   host-only memory. */
+acpp_status acpp_stdpar_should_offload(int device, int op, int elem_bytes, int is_float, uint64_t n,
+                                       const void *const *pointers, uint32_t num_pointers, int *offload);
+/* Process-wide override of the learned choice (ACPP_STDPAR_OFFLOAD_*). */
+acpp_status acpp_stdpar_set_offload_policy(int policy);
+/* *min_n = the smallest n the model offloads (for memory both sides can
+   read); UINT64_MAX when it never does, 0 when no model is compiled in. */
+acpp_status acpp_stdpar_offload_threshold(int device, int op, int elem_bytes, int is_float, uint64_t *min_n);
 
 /* Number of compute units of `device` (the value the reference reads as
    info::device::max_compute_units, numeric.hpp:154-156). */
@@ -217,6 +282,17 @@ acpp_status acpp_transform_reduce_u32(acpp_allocation_group *scratch, acpp_strea
 acpp_status acpp_transform_reduce_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *out, uint64_t init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
 acpp_status acpp_transform_reduce_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, float init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
 acpp_status acpp_transform_reduce_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *out, double init, acpp_algorithm_op op, acpp_unary_op transform, const acpp_launch_config *config);
+/* Binary transform_reduce (numeric.hpp:170-191) with any reduction of
+   plus, multiplies, minimum, maximum and a pair transform code:
+   *out = op(init, op over transform(first1[i], first2[i])); an empty range
+   leaves *out untouched (numeric.hpp:165-169).  Other reduction operators
+   give ACPP_ERROR_UNSUPPORTED. */
+acpp_status acpp_transform_reduce_binary_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first1, uint64_t n, const int32_t *first2, int32_t *out, int32_t init, acpp_algorithm_op op, acpp_binary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_binary_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first1, uint64_t n, const int64_t *first2, int64_t *out, int64_t init, acpp_algorithm_op op, acpp_binary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_binary_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first1, uint64_t n, const uint32_t *first2, uint32_t *out, uint32_t init, acpp_algorithm_op op, acpp_binary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_binary_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first1, uint64_t n, const uint64_t *first2, uint64_t *out, uint64_t init, acpp_algorithm_op op, acpp_binary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_binary_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first1, uint64_t n, const float *first2, float *out, float init, acpp_algorithm_op op, acpp_binary_op transform, const acpp_launch_config *config);
+acpp_status acpp_transform_reduce_binary_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first1, uint64_t n, const double *first2, double *out, double init, acpp_algorithm_op op, acpp_binary_op transform, const acpp_launch_config *config);
 
 /* ------------------------------------------------------------------------ */
 /* sycl::reduction over a 1-D range (handler.hpp:799-975 with              */
@@ -235,6 +311,23 @@ acpp_status acpp_reduction_u32(acpp_allocation_group *scratch, acpp_stream_t str
 acpp_status acpp_reduction_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *out, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+/* sycl::reduction(T *var, const T &identity, combiner, props)
+   (include/hipSYCL/sycl/libkernel/reduction.hpp:214-236;
+   algorithms/reduction/reduction_descriptor.hpp:36-53): with
+   initialize_to_identity *out = op(identity, fold) -- identity for an empty
+   range -- otherwise *out = op(*out, fold).  SYCL requires `identity` to be
+   an identity of the operator; it is the value an empty reduction yields
+   (for example +0.0 instead of the library's -0.0 for a float sum). */
+acpp_status acpp_reduction_identity_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *out, int8_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *out, int16_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *out, int32_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *out, int64_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *out, uint8_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *out, uint16_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *out, uint32_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_u64(acpp_allocation_group *scratch, acpp_stream_t stream, const uint64_t *first, uint64_t n, uint64_t *out, uint64_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, float identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_identity_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *out, double identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config);
 
 /* ------------------------------------------------------------------------ */
 /* Scans: single-pass decoupled look-back (decoupled_lookback_scan.hpp:     */
@@ -395,6 +488,17 @@ acpp_status acpp_reduction_multi_i64(acpp_allocation_group *scratch, acpp_stream
 acpp_status acpp_reduction_multi_u64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint64_t *const *inputs, uint64_t n, uint64_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_f32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const float *const *inputs, uint64_t n, float *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_f64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const double *const *inputs, uint64_t n, double *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+/* The same with the properties of each sycl::reduction object of the
+   parallel_for: initialize_to_identity[k] (host array of `count` flags)
+   and, when `identities` is not NULL, a given identity per reduction
+   (reduction.hpp:214-236): with the flag *outs[k] = op(identities[k],
+   fold), which an empty range leaves as identities[k]. */
+acpp_status acpp_reduction_multi_identity_i32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int32_t *const *inputs, uint64_t n, int32_t *const *outs, const acpp_algorithm_op *ops, const int32_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_u32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint32_t *const *inputs, uint64_t n, uint32_t *const *outs, const acpp_algorithm_op *ops, const uint32_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_i64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int64_t *const *inputs, uint64_t n, int64_t *const *outs, const acpp_algorithm_op *ops, const int64_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_u64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint64_t *const *inputs, uint64_t n, uint64_t *const *outs, const acpp_algorithm_op *ops, const uint64_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_f32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const float *const *inputs, uint64_t n, float *const *outs, const acpp_algorithm_op *ops, const float *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_f64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const double *const *inputs, uint64_t n, double *const *outs, const acpp_algorithm_op *ops, const double *identities, const int *initialize_to_identity, const acpp_launch_config *config);
 
 /* ------------------------------------------------------------------------ */
 /* copy_if: stream compaction                                               */

@@ -27,6 +27,11 @@ ERROR_INVALID_ARGUMENT = 1
 ERROR_UNSUPPORTED = 2
 ERROR_OUT_OF_MEMORY = 3
 ERROR_HIP = 4
+ERROR_WATCHDOG = 5
+VARIANT_BLOCK_LOOKBACK = 1
+VARIANT_NONTEMPORAL = 2
+VARIANT_TILE_PER_GROUP = 4
+VARIANT_FORCE_GIVE_UP = 8
 
 
 class LaunchConfig(ctypes.Structure):
@@ -56,9 +61,14 @@ def _signatures() -> dict[str, tuple[list, object]]:
     S["acpp_allocation_group_release"] = ([vp], c_int)
     S["acpp_allocation_group_destroy"] = ([vp], c_int)
     S["acpp_allocation_group_watchdog_trips"] = ([vp, POINTER(c_uint64)], c_int)
+    S["acpp_check_asynchronous_errors"] = ([c_int], c_int)
+    S["acpp_stdpar_should_offload"] = ([c_int, c_int, c_int, c_int, c_uint64, vp, c_uint32, POINTER(c_int)], c_int)
+    S["acpp_stdpar_set_offload_policy"] = ([c_int], c_int)
+    S["acpp_stdpar_offload_threshold"] = ([c_int, c_int, c_int, c_int, POINTER(c_uint64)], c_int)
     for tag, ct in SCALAR_TAGS.items():
         S[f"acpp_reduce_{tag}"] = ([vp, vp, vp, c_uint64, vp, ct, c_int32, cfg], c_int)
         S[f"acpp_reduction_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_int32, c_int, cfg], c_int)
+        S[f"acpp_reduction_identity_{tag}"] = ([vp, vp, vp, c_uint64, vp, ct, c_int32, c_int, cfg], c_int)
         S[f"acpp_inclusive_scan_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_int32, vp, cfg], c_int)
         S[f"acpp_exclusive_scan_{tag}"] = ([vp, vp, vp, c_uint64, vp, ct, c_int32, cfg], c_int)
     for tag, ct in SCALAR_TAGS.items():
@@ -69,6 +79,8 @@ def _signatures() -> dict[str, tuple[list, object]]:
             [vp, vp, vp, c_uint64, vp, ct, c_int32, c_int32, cfg], c_int)
     for tag, ct in DOT_TAGS.items():
         S[f"acpp_transform_reduce_dot_{tag}"] = ([vp, vp, vp, c_uint64, vp, vp, ct, cfg], c_int)
+        S[f"acpp_transform_reduce_binary_{tag}"] = (
+            [vp, vp, vp, c_uint64, vp, vp, ct, c_int32, c_int32, cfg], c_int)
     S["acpp_reduce_bf16"] = ([vp, vp, vp, c_uint64, vp, c_float, c_int32, cfg], c_int)
     S["acpp_transform_reduce_dot_bf16"] = ([vp, vp, vp, c_uint64, vp, vp, c_float, cfg], c_int)
     for tag in ("f32", "bf16"):
@@ -94,6 +106,8 @@ def _signatures() -> dict[str, tuple[list, object]]:
                 [vp, vp, vp, c_uint64, vp, c_uint32, c_int, cfg], c_int)
     for tag in ("i32", "u32", "i64", "u64", "f32", "f64"):
         S[f"acpp_reduction_multi_{tag}"] = ([vp, vp, c_uint32, vp, c_uint64, vp, vp, c_int, cfg], c_int)
+        S[f"acpp_reduction_multi_identity_{tag}"] = ([vp, vp, c_uint32, vp, c_uint64, vp, vp, vp, vp, cfg],
+                                                     c_int)
     for tag in ("i32", "u32", "i64", "u64", "f32", "f64"):
         ct = SCALAR_TAGS[tag]
         S[f"acpp_copy_if_{tag}"] = ([vp, vp, vp, c_uint64, vp, c_int, ct, ct, vp, cfg], c_int)

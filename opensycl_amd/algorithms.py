@@ -117,10 +117,21 @@ class Queue:
         return s.cuda_stream
 
     def wait(self) -> None:
+        """Wait for the queue's work, then raise the device's asynchronous
+        errors (queue::wait_and_throw, include/hipSYCL/sycl/queue.hpp:
+        378-383): AcppError with status ERROR_WATCHDOG if a kernel gave up
+        waiting for another work-group, whose results are then wrong."""
         if isinstance(self.stream, int):
             _torch().cuda.synchronize(self.device)
         else:
             self.stream.synchronize()
+        check_asynchronous_errors(self.device)
+
+
+def check_asynchronous_errors(device: int = 0) -> None:
+    """acpp_check_asynchronous_errors: raise (once) for a watchdog give-up
+    on `device` since the last check.  Non-blocking: synchronise first."""
+    check(_abi.load().acpp_check_asynchronous_errors(device))
 
 
 class AllocationCache:
@@ -141,8 +152,8 @@ class AllocationCache:
 
     def close(self) -> None:
         if self._h:
-            check(_abi.load().acpp_allocation_cache_destroy(self._h))
-            self._h = None
+            h, self._h = self._h, None
+            check(_abi.load().acpp_allocation_cache_destroy(h))
 
     def __del__(self):
         try:
@@ -160,6 +171,8 @@ class AllocationGroup:
         check(lib.acpp_allocation_group_create(cache.handle, device, ctypes.byref(h)))
         self._h = h
         self._cache = cache
+        self._plans = 0             # live SegmentedPlans captured from this group
+        self._close_pending = False
 
     @property
     def handle(self):
@@ -183,9 +196,16 @@ class AllocationGroup:
         return v.value
 
     def close(self) -> None:
+        """Return the group's regions to the cache.  While a SegmentedPlan
+        captured from this group is alive, its graph still uses them: the
+        release then waits until the last such plan is closed."""
+        if self._plans > 0:
+            self._close_pending = True
+            return
         if self._h:
-            check(_abi.load().acpp_allocation_group_destroy(self._h))
-            self._h = None
+            h, self._h = self._h, None
+            # destroyed even when it reports an asynchronous error
+            check(_abi.load().acpp_allocation_group_destroy(h))
 
     def __enter__(self):
         return self
@@ -220,6 +240,23 @@ def _check_contig(*ts):
             raise ValueError("tensors must be contiguous")
 
 
+def _check_out(out, dtype, count: int, what: str) -> None:
+    """The C-ABI receives raw pointers only, so an output too short or of a
+    narrower type would be written past its end: reject it before launch."""
+    if out is None:
+        raise ValueError(f"{what}: output is missing")
+    if out.dtype != dtype:
+        raise TypeError(f"{what}: output must be {dtype}, not {out.dtype}")
+    if out.numel() < count:
+        raise ValueError(f"{what}: output holds {out.numel()} elements, needs {count}")
+
+
+def _acc_dtype(x):
+    """Result type of a reduction over x: float32 for bf16 input."""
+    torch = _torch()
+    return torch.float32 if x.dtype == torch.bfloat16 else x.dtype
+
+
 def _scalar(tag: str, v):
     ct = _abi.SCALAR_TAGS.get(tag, ctypes.c_float)
     return ct(v)
@@ -228,6 +265,7 @@ def _scalar(tag: str, v):
 def reduce(q: Queue, scratch: AllocationGroup, x, out, init=0, op=PLUS, config=None) -> None:
     """algorithms::reduce (numeric.hpp:226-249): out[0] = op(init, fold(x))."""
     _check_contig(x, out)
+    _check_out(out, _acc_dtype(x), 1, "reduce")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_reduce_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), _scalar(tag, init),
@@ -241,19 +279,44 @@ def reduce_compensated(q: Queue, scratch: AllocationGroup, x, out, init=0.0,
     tag = _tag(x.dtype)
     if tag not in ("f32", "bf16"):
         raise TypeError("compensated sums take float32 or bfloat16 input")
+    _check_out(out, _torch().float32, 1, "reduce_compensated")
     fn = getattr(_abi.load(), f"acpp_reduce_compensated_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), ctypes.c_float(init),
              _cfg(config)))
 
 
+# Pair transform codes of the binary transform_reduce (== acpp_binary_op)
+B_MULTIPLIES, B_PLUS, B_MINUS, B_ABS_DIFF, B_SQ_DIFF, B_MIN, B_MAX, B_BIT_AND, B_BIT_OR, B_BIT_XOR = range(10)
+_BINARY_NAMES = {
+    "multiplies": B_MULTIPLIES, "plus": B_PLUS, "minus": B_MINUS, "abs_diff": B_ABS_DIFF,
+    "sq_diff": B_SQ_DIFF, "minimum": B_MIN, "maximum": B_MAX, "bit_and": B_BIT_AND, "bit_or": B_BIT_OR,
+    "bit_xor": B_BIT_XOR,
+}
+
+
 def transform_reduce(q: Queue, scratch: AllocationGroup, x1, x2, out, init=0,
-                     compensated: bool = False, config=None) -> None:
-    """Binary transform_reduce with plus / multiplies (numeric.hpp:204-223)."""
+                     compensated: bool = False, config=None, reduce_op=PLUS,
+                     transform=B_MULTIPLIES) -> None:
+    """Binary transform_reduce (numeric.hpp:170-223): out[0] = reduce_op(init,
+    reduce_op over transform(x1[i], x2[i])).  The default (plus, multiplies)
+    is the dot product, also for bf16 and in the compensated form; other
+    pairs (reduce_op: plus, multiplies, minimum, maximum; transform: a pair
+    transform code or name) take 32/64-bit integers, float32 and float64."""
     _check_contig(x1, x2, out)
     if x1.numel() > x2.numel() or x1.dtype != x2.dtype:
         raise ValueError("second range must have the first range's type and length")
     tag = _tag(x1.dtype)
+    _check_out(out, _torch().float32 if compensated else _acc_dtype(x1), 1, "transform_reduce")
+    code = _BINARY_NAMES[transform] if isinstance(transform, str) else int(transform)
+    op = _op(reduce_op)
     lib = _abi.load()
+    if (op, code) != (PLUS, B_MULTIPLIES):
+        if compensated or tag == "bf16":
+            raise TypeError("compensated and bf16 transform_reduce take plus with multiplies only")
+        fn = getattr(lib, f"acpp_transform_reduce_binary_{tag}")
+        check(fn(scratch.handle, q.handle, _ptr(x1), x1.numel(), _ptr(x2), _ptr(out), _scalar(tag, init), op,
+                 code, _cfg(config)))
+        return
     if compensated:
         if tag not in ("f32", "bf16"):
             raise TypeError("compensated dot products take float32 or bfloat16 input")
@@ -271,6 +334,7 @@ def transform_reduce_unary(q: Queue, scratch: AllocationGroup, x, out, init=0, o
     """Unary transform_reduce (numeric.hpp:193-212):
     out[0] = op(init, fold_op(transform(x[i]))); empty x leaves out as is."""
     _check_contig(x, out)
+    _check_out(out, x.dtype, 1, "transform_reduce")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_transform_reduce_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), _scalar(tag, init),
@@ -278,21 +342,34 @@ def transform_reduce_unary(q: Queue, scratch: AllocationGroup, x, out, init=0, o
 
 
 def reduction(q: Queue, scratch: AllocationGroup, x, out, op=PLUS,
-              initialize_to_identity: bool = False, config=None) -> None:
-    """parallel_for(range{n}, sycl::reduction(out, op[, initialize_to_identity]))."""
+              initialize_to_identity: bool = False, config=None, identity=None) -> None:
+    """parallel_for(range{n}, sycl::reduction(out, [identity,] op
+    [, initialize_to_identity])) whose kernel combines x[i]
+    (reduction.hpp:165-236): with initialize_to_identity out[0] = op(identity,
+    fold), else op(out[0], fold).  Without a given identity the operator's
+    own is used."""
     _check_contig(x, out)
+    _check_out(out, x.dtype, 1, "reduction")
     tag = _tag(x.dtype)
-    fn = getattr(_abi.load(), f"acpp_reduction_{tag}")
-    check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out), _op(op),
-             1 if initialize_to_identity else 0, _cfg(config)))
+    lib = _abi.load()
+    flag = 1 if initialize_to_identity else 0
+    if identity is None:
+        check(getattr(lib, f"acpp_reduction_{tag}")(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(out),
+                                                    _op(op), flag, _cfg(config)))
+    else:
+        check(getattr(lib, f"acpp_reduction_identity_{tag}")(scratch.handle, q.handle, _ptr(x), x.numel(),
+                                                             _ptr(out), _scalar(tag, identity), _op(op),
+                                                             flag, _cfg(config)))
 
 
 def reduction_multi(q: Queue, scratch: AllocationGroup, inputs, outs, ops,
-                    initialize_to_identity: bool = False, config=None) -> None:
+                    initialize_to_identity=False, config=None, identities=None) -> None:
     """Up to four sycl::reduction objects in one parallel_for
     (handler.hpp:799-975): outs[k][0] = fold of inputs[k] with ops[k]
     (starting from the identity, or from outs[k][0]).  Inputs share one
-    length and element type; they may be the same tensor."""
+    length and element type; they may be the same tensor.
+    initialize_to_identity: one flag for all, or one per reduction;
+    identities: None, or one given identity per reduction."""
     k = len(inputs)
     if not (1 <= k <= 4) or len(outs) != k or len(ops) != k:
         raise ValueError("one to four (input, output, operator) triples")
@@ -302,18 +379,34 @@ def reduction_multi(q: Queue, scratch: AllocationGroup, inputs, outs, ops,
         _check_contig(x, o)
         if x.numel() != n or _tag(x.dtype) != tag or _tag(o.dtype) != tag:
             raise ValueError("inputs must share length and element type with the outputs")
+        _check_out(o, x.dtype, 1, "reduction_multi")
     arr_in = (ctypes.c_void_p * k)(*[_ptr(x) for x in inputs])
     arr_out = (ctypes.c_void_p * k)(*[_ptr(o) for o in outs])
     arr_ops = (ctypes.c_int32 * k)(*[_op(o) for o in ops])
-    fn = getattr(_abi.load(), f"acpp_reduction_multi_{tag}")
-    check(fn(scratch.handle, q.handle, k, arr_in, n, arr_out, arr_ops, int(bool(initialize_to_identity)),
-             _cfg(config)))
+    lib = _abi.load()
+    if identities is None and isinstance(initialize_to_identity, bool):
+        fn = getattr(lib, f"acpp_reduction_multi_{tag}")
+        check(fn(scratch.handle, q.handle, k, arr_in, n, arr_out, arr_ops, int(initialize_to_identity),
+                 _cfg(config)))
+        return
+    flags = list(initialize_to_identity) if not isinstance(initialize_to_identity, bool) \
+        else [initialize_to_identity] * k
+    if len(flags) != k or (identities is not None and len(identities) != k):
+        raise ValueError("one initialize_to_identity flag and identity per reduction")
+    arr_flags = (ctypes.c_int * k)(*[int(bool(f)) for f in flags])
+    arr_id = None
+    if identities is not None:
+        ct = _abi.SCALAR_TAGS[tag]
+        arr_id = (ct * k)(*identities)
+    fn = getattr(lib, f"acpp_reduction_multi_identity_{tag}")
+    check(fn(scratch.handle, q.handle, k, arr_in, n, arr_out, arr_ops, arr_id, arr_flags, _cfg(config)))
 
 
 def inclusive_scan(q: Queue, scratch: AllocationGroup, x, d, op=PLUS, init=None,
                    config=None) -> None:
     """algorithms::inclusive_scan (numeric.hpp:253-279)."""
     _check_contig(x, d)
+    _check_out(d, x.dtype, x.numel(), "inclusive_scan")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_inclusive_scan_{tag}")
     init_p = None
@@ -328,6 +421,7 @@ def exclusive_scan(q: Queue, scratch: AllocationGroup, x, d, init=0, op=PLUS,
                    config=None) -> None:
     """algorithms::exclusive_scan (numeric.hpp:281-297)."""
     _check_contig(x, d)
+    _check_out(d, x.dtype, x.numel(), "exclusive_scan")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_exclusive_scan_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _scalar(tag, init),
@@ -339,6 +433,7 @@ def transform_inclusive_scan(q: Queue, scratch: AllocationGroup, x, d, op=PLUS,
     """algorithms::transform_inclusive_scan (numeric.hpp:299-317): the
     inclusive scan of transform(x[i])."""
     _check_contig(x, d)
+    _check_out(d, x.dtype, x.numel(), "transform_inclusive_scan")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_transform_inclusive_scan_{tag}")
     init_p = None
@@ -354,6 +449,7 @@ def transform_exclusive_scan(q: Queue, scratch: AllocationGroup, x, d, init=0, o
     """algorithms::transform_exclusive_scan (numeric.hpp:319-327): the
     exclusive scan of transform(x[i]) starting from init."""
     _check_contig(x, d)
+    _check_out(d, x.dtype, x.numel(), "transform_exclusive_scan")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_transform_exclusive_scan_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _scalar(tag, init),
@@ -388,6 +484,8 @@ def atomic_fetch_strided(q: Queue, scratch: AllocationGroup, op: str, values, ta
     tag = _tag(values.dtype)
     if values.dtype != target.dtype:
         raise TypeError("values and target must have the same type")
+    if target.numel() < 1:
+        raise ValueError("target must hold at least one slot")
     fn = getattr(_abi.load(), f"acpp_atomic_fetch_{op}_strided_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(values), values.numel(), _ptr(target), target.numel(),
              strategy, _cfg(config)))
@@ -408,8 +506,9 @@ def copy_if(q: Queue, scratch: AllocationGroup, x, d, pred: int, operand=0, mask
     d[0..count) = the x[i] with pred(x[i]); num_copied[0] = count (a uint64
     device tensor, optional)."""
     _check_contig(x, d)
-    if num_copied is not None and _tag(num_copied.dtype) != "u64":
-        raise TypeError("num_copied must be a uint64 tensor")
+    _check_out(d, x.dtype, x.numel(), "copy_if")
+    if num_copied is not None:
+        _check_out(num_copied, _torch().uint64, 1, "copy_if num_copied")
     tag = _tag(x.dtype)
     if tag not in _COPY_IF_TAGS:
         raise TypeError(f"copy_if takes 4- or 8-byte elements (i32, u32, i64, u64, f32, f64), not {tag}")
@@ -425,6 +524,9 @@ def copy_if_stencil(q: Queue, scratch: AllocationGroup, x, stencil, d, num_copie
     _check_contig(x, d, stencil)
     if stencil.numel() < x.numel() or stencil.element_size() != 1:
         raise ValueError("stencil must be a one-byte tensor at least as long as x")
+    _check_out(d, x.dtype, x.numel(), "copy_if_stencil")
+    if num_copied is not None:
+        _check_out(num_copied, _torch().uint64, 1, "copy_if_stencil num_copied")
     tag = _tag(x.dtype)
     if tag not in _COPY_IF_TAGS:
         raise TypeError(f"copy_if takes 4- or 8-byte elements (i32, u32, i64, u64, f32, f64), not {tag}")
@@ -439,6 +541,8 @@ def copy_if_stencil(q: Queue, scratch: AllocationGroup, x, stencil, d, num_copie
 def _seg_check(x, offsets):
     if _tag(offsets.dtype) != "u64":
         raise TypeError("offsets must be uint64")
+    if offsets.numel() < 1:
+        raise ValueError("offsets must hold num_segments + 1 entries")
     _check_contig(x, offsets)
 
 
@@ -447,6 +551,7 @@ def segmented_inclusive_scan(q: Queue, scratch: AllocationGroup, x, d, offsets, 
     """d[i] = inclusive scan of x within the segment [offsets[s], offsets[s+1])."""
     _seg_check(x, offsets)
     _check_contig(d)
+    _check_out(d, x.dtype, x.numel(), "segmented_inclusive_scan")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_segmented_inclusive_scan_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), _ptr(offsets),
@@ -458,6 +563,7 @@ def segmented_reduce(q: Queue, scratch: AllocationGroup, x, offsets, out, init=0
     """out[s] = op(init, fold of x over [offsets[s], offsets[s+1]))."""
     _seg_check(x, offsets)
     _check_contig(out)
+    _check_out(out, x.dtype, offsets.numel() - 1, "segmented_reduce")
     tag = _tag(x.dtype)
     fn = getattr(_abi.load(), f"acpp_segmented_reduce_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(offsets), offsets.numel() - 1,
@@ -466,38 +572,56 @@ def segmented_reduce(q: Queue, scratch: AllocationGroup, x, offsets, out, init=0
 
 class SegmentedPlan:
     """A segmented scan or reduction captured into a hipGraph once and
-    replayed with ``launch``."""
+    replayed with ``launch``.
 
-    def __init__(self, handle):
+    The captured graph holds raw pointers to the input, output and offsets
+    tensors and to scratch regions of the allocation group, so the plan
+    keeps references to all of them (and to the group's cache) for as long
+    as it lives: a replay never touches freed or reissued memory."""
+
+    def __init__(self, handle, scratch, keep=()):
         self._h = handle
+        self._scratch = scratch
+        scratch._plans += 1
+        self._keep = tuple(keep)
 
     @staticmethod
     def inclusive_scan(scratch: AllocationGroup, x, d, offsets, op=PLUS, config=None):
         _seg_check(x, offsets)
+        _check_contig(d)
+        _check_out(d, x.dtype, x.numel(), "segmented plan inclusive_scan")
         tag = _tag(x.dtype)
         h = ctypes.c_void_p()
         fn = getattr(_abi.load(), f"acpp_segmented_plan_inclusive_scan_{tag}")
         check(fn(scratch.handle, _ptr(x), x.numel(), _ptr(d), _ptr(offsets), offsets.numel() - 1,
                  _op(op), _cfg(config), ctypes.byref(h)))
-        return SegmentedPlan(h)
+        return SegmentedPlan(h, scratch, (x, d, offsets))
 
     @staticmethod
     def reduce(scratch: AllocationGroup, x, offsets, out, init=0, op=PLUS, config=None):
         _seg_check(x, offsets)
+        _check_contig(out)
+        _check_out(out, x.dtype, offsets.numel() - 1, "segmented plan reduce")
         tag = _tag(x.dtype)
         h = ctypes.c_void_p()
         fn = getattr(_abi.load(), f"acpp_segmented_plan_reduce_{tag}")
         check(fn(scratch.handle, _ptr(x), x.numel(), _ptr(offsets), offsets.numel() - 1,
                  _ptr(out), _scalar(tag, init), _op(op), _cfg(config), ctypes.byref(h)))
-        return SegmentedPlan(h)
+        return SegmentedPlan(h, scratch, (x, out, offsets))
 
     def launch(self, q: Queue) -> None:
         check(_abi.load().acpp_segmented_plan_launch(self._h, q.handle))
 
     def close(self) -> None:
         if self._h:
-            check(_abi.load().acpp_segmented_plan_destroy(self._h))
-            self._h = None
+            h, self._h = self._h, None
+            check(_abi.load().acpp_segmented_plan_destroy(h))
+        self._keep = ()
+        g, self._scratch = self._scratch, None
+        if g is not None:
+            g._plans -= 1
+            if g._close_pending and g._plans == 0:
+                g.close()
 
     def __del__(self):
         try:

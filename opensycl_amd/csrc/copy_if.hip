@@ -48,6 +48,7 @@ struct params {
   uint32_t num_tiles;
   uint32_t epoch;
   uint32_t ticket_base;
+  int force_give_up;  // ACPP_VARIANT_FORCE_GIVE_UP (tests of the error path)
   sync_header *sync;
   uint64_t *status;
 };
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
   if (w == 0 && tile != 0) {
     wait_own_stores();  // the tile's count is out before the first poll
     const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t, scan_detail::lookback_slots>(
-        p.status, tile, p.epoch, &p.sync->watchdog_trips);
+        p.status, tile, p.epoch, p.sync, p.force_give_up != 0);
     if (lane == 0) {
       if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, p.epoch, st_prefix, e + agg);
       s_excl = e;
@@ -288,23 +289,9 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   const size_t used = sizeof(sync_header) + num_tiles * status_stride_words * sizeof(uint64_t);
   st = group_obtain_sync(g, used, stream, &region, &cap, &rs);
   if (st != ACPP_SUCCESS) return st;
-  const bool capturing = stream_is_capturing(stream);
-  uint32_t epoch = 1, ticket_base = 0;
-  if (capturing) {
-    st = fill_zero(region, used, stream);
-    if (st != ACPP_SUCCESS) return st;
-    rs->mode = region_state::graph;
-  } else {
-    if (rs->mode == region_state::graph || rs->epoch >= max_epoch) {
-      st = fill_zero(region, cap, stream);
-      if (st != ACPP_SUCCESS) return st;
-      rs->epoch = 0;
-      rs->ticket_base = 0;
-    }
-    rs->mode = region_state::host_managed;
-    epoch = rs->epoch + 1;
-    ticket_base = rs->ticket_base;
-  }
+  region_launch rl;
+  st = begin_region_launch(g, region, used, cap, rs, stream, &rl);
+  if (st != ACPP_SUCCESS) return st;
   params<T> p;
   p.in = in;
   p.stencil = stencil;
@@ -320,20 +307,16 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   p.vec_io = (reinterpret_cast<uintptr_t>(in) & 3) == 0;
   p.out_vec = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
-  p.epoch = epoch;
-  p.ticket_base = ticket_base;
+  p.epoch = rl.epoch;
+  p.ticket_base = rl.ticket_base;
+  p.force_give_up = (cfg && (cfg->flags & ACPP_VARIANT_FORCE_GIVE_UP)) ? 1 : 0;
   p.sync = static_cast<sync_header *>(region);
   p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
   const size_t stage_bytes = static_cast<size_t>(tile_elems) * sizeof(T);  // 128 KiB at 1024 x 8
   hipLaunchKernelGGL((copy_if_kernel<T, rows<T>()>), dim3(static_cast<uint32_t>(num_tiles)), dim3(wg),
                      stage_bytes, stream, p);
   st = check_launch(name);
-  if (st == ACPP_SUCCESS && !capturing) {
-    rs->epoch = epoch;
-    rs->ticket_base = ticket_base + static_cast<uint32_t>(num_tiles);
-  } else if (st != ACPP_SUCCESS) {
-    rs->mode = region_state::graph;
-  }
+  end_region_launch(rs, rl, static_cast<uint32_t>(num_tiles), st);
   return st == ACPP_SUCCESS ? clear_error() : st;
 }
 

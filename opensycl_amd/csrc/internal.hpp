@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <string>
@@ -49,11 +50,27 @@ struct sync_header {
   uint32_t epoch;           // unused by eager launches (epochs are kernel arguments)
   uint32_t tile_counter;    // dynamic tile ticket, monotonic across launches
   uint32_t done_counter;    // group ticket of the single-launch reductions
-  uint32_t watchdog_trips;  // look-backs that gave up waiting (never expected)
-  uint32_t pad[60];         // header occupies 256 bytes
+  uint32_t watchdog_trips;  // waits that gave up (never expected)
+  // Device address of the pinned host word that makes a give-up visible to
+  // the host without a device read (host_trip_flag in runtime.cpp); written
+  // into every region whenever the region is cleared (reset_sync_region).
+  uint32_t *host_trip_flag;
+  uint32_t pad[58];         // header occupies 256 bytes
 };
 
 static_assert(sizeof(sync_header) == 256, "sync header is 256 bytes");
+static_assert(offsetof(sync_header, host_trip_flag) == 16, "trip flag pointer is the second 16-byte chunk");
+
+// A wait inside a kernel gave up: count it in the region and raise the
+// host-visible flag, which the host reports as ACPP_ERROR_WATCHDOG at its
+// next synchronisation point (acpp_check_asynchronous_errors).  The flag is
+// written with a plain system-scope store (no read-modify-write on host
+// memory).
+__device__ __forceinline__ void report_trip(sync_header *sync) {
+  __hip_atomic_fetch_add(&sync->watchdog_trips, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t *flag = sync->host_trip_flag;
+  if (flag) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Waiting inside a decoupled look-back.
 //   - Backoff: a poll round that finds an unfinished predecessor sleeps
@@ -64,10 +81,14 @@ static_assert(sizeof(sync_header) == 256, "sync header is 256 bytes");
 //     over 2e5 polls per look-back on MI355X without it.
 //   - Watchdog: the wait is bounded in wall-clock time (the steady 100 MHz
 //     counter), not in polls, so a slow but live chain never trips it.
-//     After lookback_timeout_ticks the look-back gives up, counts a trip in
-//     sync_header::watchdog_trips and continues with what it has (the
-//     results are then wrong; acpp_allocation_group_watchdog_trips reports
-//     it).  It only guards against a device that stops making progress.
+//     After lookback_timeout_ticks the look-back gives up, reports a trip
+//     (report_trip) and continues with what it has, so every kernel still
+//     ends; its results are then wrong, and the host reports the trip as an
+//     error (ACPP_ERROR_WATCHDOG) at its next synchronisation point.  It
+//     only guards against a device that stops making progress.
+//   - ACPP_VARIANT_FORCE_GIVE_UP (a launch-config flag for tests) makes
+//     every such wait give up at once, so the error path can be exercised
+//     with kernels that still end normally.
 constexpr uint64_t lookback_timeout_ticks = 1ull << 28;  // ~2.7 s at 100 MHz
 constexpr uint32_t lookback_max_backoff = 64;
 
@@ -132,6 +153,45 @@ struct region_state {
 
 // Obtain ordinary scratch from the group.
 acpp_status group_obtain(acpp_allocation_group *g, size_t bytes, void **out);
+// Epoch and ticket base of one launch on a sync region.  Eager launches
+// continue the region's host-side count.  Under stream capture the graph may
+// be replayed any number of times, so the captured work clears the region
+// first (with a kernel) and always runs with epoch 1 and ticket base 0.
+struct region_launch {
+  uint32_t epoch = 1;
+  uint32_t ticket_base = 0;
+  bool capturing = false;
+};
+// Prepare `region` (of `capacity` bytes, `used` of them needed) for a launch.
+acpp_status begin_region_launch(acpp_allocation_group *g, void *region, size_t used, size_t capacity,
+                                region_state *rs, hipStream_t stream, region_launch *out);
+// The same for a region that belongs to no allocation group (the library's
+// static no-allocation scratch): `trip_flag` (may be NULL) is stored in the
+// header whenever the region is cleared.
+acpp_status begin_region_launch_flag(uint32_t *trip_flag, void *region, size_t used, size_t capacity,
+                                     region_state *rs, hipStream_t stream, region_launch *out);
+
+// A sync region handed to a launcher instead of one from an allocation
+// group.  The caller makes its device current and knows its CU count.
+struct fixed_region {
+  void *ptr;            // device address
+  size_t capacity;      // bytes
+  region_state *rs;
+  uint32_t *trip_flag;  // host trip flag of the device, or NULL
+  int compute_units;
+};
+// Record the launch: `tickets` tile tickets drawn; a failed launch makes the
+// region start over next time (it may or may not have run).
+void end_region_launch(region_state *rs, const region_launch &l, uint32_t tickets, acpp_status launch_status);
+// Clear the first `bytes` of a sync region with a kernel and store the
+// device's host trip flag address in its header.
+acpp_status reset_sync_region(void *region, size_t bytes, int device, hipStream_t stream);
+acpp_status reset_sync_region_flag(void *region, size_t bytes, uint32_t *trip_flag, hipStream_t stream);
+// Pinned, device-visible host word of `device` raised by report_trip
+// (allocated on first use), and the same without allocating (NULL if no
+// call has allocated it yet).
+uint32_t *host_trip_flag(int device);
+uint32_t *host_trip_flag_if_allocated(int device);
 // Obtain a sync region of at least `bytes` (header included).  Fresh regions
 // are zeroed on `stream` before first use.  `capacity_bytes` receives the
 // region's real size, `state` its host-side state.

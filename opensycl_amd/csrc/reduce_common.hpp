@@ -18,10 +18,17 @@
 //     across waves;
 //   - every group stores its partial, then takes a ticket; the group that
 //     draws the last ticket folds all partials in a fixed order, writes
-//     *out and resets the ticket to zero for the next launch.
+//     *out and resets the ticket to zero for the next launch;
+//   - the partials are self-validating: each is stored with the launch's
+//     epoch tag (the scan's status-word layout), and the last group waits
+//     until every partial carries the current tag.  So the result does not
+//     depend on the partial's store becoming visible before the ticket:
+//     no fence and no ordering between the two accesses is needed (a
+//     release fence per group would write back the XCD's L2 on gfx950).
 //   Floating-point results are deterministic for a given launch geometry.
 #pragma once
 
+#include "binary.hpp"
 #include "internal.hpp"
 #include "unary.hpp"
 #include "acpp/device/wave.hpp"
@@ -121,6 +128,9 @@ enum final_mode : int {
   final_with_init = 0,     // *out = op(init, r)                (algorithms::reduce)
   final_accumulate = 1,    // *out = op(*out, r)                (sycl::reduction)
   final_identity_init = 2, // *out = r  (== op(identity, r))   (initialize_to_identity)
+  final_given_identity = 3, // *out = op(init, r), also for an empty range
+                            // (sycl::reduction with a user identity and
+                            // initialize_to_identity: init = that identity)
 };
 
 template <class T, class M, class OutT>
@@ -131,27 +141,79 @@ struct params {
   uint64_t nvec;      // 16-byte chunks in the aligned region
   uint64_t nscalar;   // head + tail elements handled by the scalar loop
   uint64_t tail_start;
-  typename M::value *partials;
+  uint64_t *partials;  // one tagged slot per work-group (partial_slot_words)
   sync_header *sync;
   OutT *out;
   typename elem_traits<T>::work init;
   int mode;
-  int unary;  // element transform (unary.hpp) of the unary transform_reduce
+  int unary;          // element transform (unary.hpp) of the unary transform_reduce
+  int binary;         // pair transform (binary.hpp) of the binary transform_reduce
+  uint32_t epoch;     // tag of this launch's partials (>= 1)
+  int force_give_up;  // ACPP_VARIANT_FORCE_GIVE_UP (tests of the error path)
 };
 
-// Device-scope (cross-XCD coherent) store / load of a group partial.
-template <class V>
-__device__ __forceinline__ void store_partial(V *dst, V v) {
-  static_assert(sizeof(V) == 4 || sizeof(V) == 8, "partials are 32 or 64 bits");
-  using U = std::conditional_t<sizeof(V) == 4, uint32_t, uint64_t>;
-  __hip_atomic_store(reinterpret_cast<U *>(dst), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+// ---------------------------------------------------------------------------
+// Tagged group partials.  A partial of 32 bits is one 64-bit word
+// tag << 32 | bits, one of 64 bits two such words (low half, high half),
+// both carrying the tag, with tag = status_tag(epoch, 0, 2) as in the scans'
+// status words.  Words left by earlier launches of any kernel on the same
+// scratch region carry older epochs and never read as valid.  The words are
+// written and read with relaxed device-scope (cross-XCD coherent) atomics.
+// ---------------------------------------------------------------------------
+constexpr int partial_slot_words = 2;  // 16 bytes per work-group
+
+__device__ __forceinline__ uint64_t partial_tag(uint32_t epoch) {
+  return static_cast<uint64_t>(status_tag(epoch, 0, 2)) << 32;
 }
+
 template <class V>
-__device__ __forceinline__ V load_partial(V *src) {
-  using U = std::conditional_t<sizeof(V) == 4, uint32_t, uint64_t>;
-  return __builtin_bit_cast(V, __hip_atomic_load(reinterpret_cast<U *>(src), __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT));
+__device__ __forceinline__ void publish_partial(uint64_t *slots, uint64_t idx, uint32_t epoch, V v) {
+  static_assert(sizeof(V) == 4 || sizeof(V) == 8, "partials are 32 or 64 bits");
+  uint64_t *slot = slots + idx * partial_slot_words;
+  const uint64_t tag = partial_tag(epoch);
+  if constexpr (sizeof(V) == 4) {
+    __hip_atomic_store(slot, tag | __builtin_bit_cast(uint32_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+    __hip_atomic_store(slot, tag | (bits & 0xffffffffu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// True (and v set) once the slot holds this launch's partial.
+template <class V>
+__device__ __forceinline__ bool read_partial(uint64_t *slots, uint64_t idx, uint32_t epoch, V &v) {
+  uint64_t *slot = slots + idx * partial_slot_words;
+  const uint64_t tag = partial_tag(epoch);
+  const uint64_t w0 = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if constexpr (sizeof(V) == 4) {
+    if ((w0 & ~0xffffffffull) != tag) return false;
+    v = __builtin_bit_cast(V, static_cast<uint32_t>(w0));
+  } else {
+    const uint64_t w1 = __hip_atomic_load(slot + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((w0 & ~0xffffffffull) != tag || (w1 & ~0xffffffffull) != tag) return false;
+    v = __builtin_bit_cast(V, (w1 << 32) | (w0 & 0xffffffffu));
+  }
+  return true;
+}
+
+// The partial of work-group idx, waited for (with backoff) until it carries
+// this launch's tag.  A wait that runs out of time, or any wait under
+// force_give_up, reports a trip and yields `missing` instead.
+template <class V>
+__device__ V wait_partial(uint64_t *slots, uint64_t idx, uint32_t epoch, sync_header *sync, bool force_give_up,
+                          V missing) {
+  V v;
+  if (!force_give_up && read_partial(slots, idx, epoch, v)) return v;
+  lookback_wait wait;
+  wait.begin();
+  for (;;) {
+    if (force_give_up || wait.failed_round()) {
+      report_trip(sync);
+      return missing;
+    }
+    if (read_partial(slots, idx, epoch, v)) return v;
+  }
 }
 
 template <bool NT, class T>
@@ -185,9 +247,13 @@ __device__ __forceinline__ void unpack(u32x4 q, W (&out)[VEC]) {
 
 // XF: apply the element transform p.unary to every element before it is
 // combined (unary transform_reduce); never together with DOT.
-template <class T, class M, class Op, class OutT, int UNROLL, bool DOT, bool NT = false, bool XF = false>
+// BX (with DOT): combine p.binary(x[i], y[i]) (binary.hpp) with Op instead
+// of the dot product's plus of products (binary transform_reduce).
+template <class T, class M, class Op, class OutT, int UNROLL, bool DOT, bool NT = false, bool XF = false,
+          bool BX = false>
 __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   static_assert(!(DOT && XF), "a dot product takes no element transform");
+  static_assert(!BX || DOT, "a pair transform needs two operands");
   using W = typename elem_traits<T>::work;
   using V = typename M::value;
   constexpr int VEC = 16 / sizeof(T);
@@ -240,7 +306,13 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     for (int u = 0; u < UNROLL; ++u) {
       W xa[VEC];
       unpack<T, W, VEC>(qa[u], xa);
-      if constexpr (DOT) {
+      if constexpr (DOT && BX) {
+        W xb[VEC], t[VEC];
+        unpack<T, W, VEC>(qb[u], xb);
+        binary_detail::apply_row<T, W, VEC>(p.binary, xa, xb, t);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc[u] = M::insert(acc[u], t[k]);
+      } else if constexpr (DOT) {
         W xb[VEC];
         unpack<T, W, VEC>(qb[u], xb);
 #pragma unroll
@@ -255,7 +327,13 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     W xa[VEC];
     unpack<T, W, VEC>(load16<NT>(va, c), xa);
     if constexpr (XF) unary_detail::apply_row<T>(p.unary, xa);
-    if constexpr (DOT) {
+    if constexpr (DOT && BX) {
+      W xb[VEC], t[VEC];
+      unpack<T, W, VEC>(load16_a4<NT>(vb, c), xb);
+      binary_detail::apply_row<T, W, VEC>(p.binary, xa, xb, t);
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) acc[0] = M::insert(acc[0], t[k]);
+    } else if constexpr (DOT) {
       W xb[VEC];
       unpack<T, W, VEC>(load16_a4<NT>(vb, c), xb);
 #pragma unroll
@@ -270,7 +348,9 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     const uint64_t idx = (s < p.head) ? s : p.tail_start + (s - p.head);
     W xa = elem_traits<T>::to_work(p.a[idx]);
     if constexpr (XF) xa = unary_detail::apply_one<T>(p.unary, xa);
-    if constexpr (DOT) {
+    if constexpr (DOT && BX) {
+      acc[0] = M::insert(acc[0], binary_detail::apply_one<T, W>(p.binary, xa, elem_traits<T>::to_work(p.b[idx])));
+    } else if constexpr (DOT) {
       acc[0] = M::insert2(acc[0], xa, elem_traits<T>::to_work(p.b[idx]));
     } else {
       acc[0] = M::insert(acc[0], xa);
@@ -284,13 +364,12 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   __shared__ int s_last;
   V bv = block_reduce<M>(v, s_waves);
   if (threadIdx.x == 0) {
-    // The partial is written with a device-scope store and the ticket is
-    // taken only once that store has completed; the last group reads the
-    // partials with device-scope loads.  No release/acquire fence is needed,
-    // which on gfx950 would write back / invalidate this XCD's L2 once per
-    // work-group.
-    store_partial(p.partials + blockIdx.x, bv);
-    __builtin_amdgcn_s_waitcnt(0);
+    // Tagged partial, then the ticket.  The last group waits for each
+    // partial's tag, so the order in which the two become visible does not
+    // matter; waiting for the store first (vmcnt) only makes such a wait
+    // rare.
+    publish_partial(p.partials, blockIdx.x, p.epoch, bv);
+    wait_own_stores();
     uint32_t prev = __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
     s_last = (prev == gridDim.x - 1);
@@ -300,12 +379,13 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
 
   // Last work-group: fold the partials in a fixed order.
   const int nparts = gridDim.x;
+  const bool force = p.force_give_up != 0;
   if constexpr (M::compensated) {
     // Cross-group combine of (sum, error) pairs in FP64.
     __shared__ double s_dwaves[max_waves];
     double d = -0.0;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-      V q = load_partial(p.partials + i);
+      V q = wait_partial<V>(p.partials, i, p.epoch, p.sync, force, M::identity());
       d += static_cast<double>(q.s) + static_cast<double>(q.c);
     }
     d = block_reduce<double_plus<double>>(d, s_dwaves);
@@ -318,11 +398,12 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     }
   } else {
     V r = M::identity();
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) r = M::apply(r, load_partial(p.partials + i));
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+      r = M::apply(r, wait_partial<V>(p.partials, i, p.epoch, p.sync, force, M::identity()));
     __syncthreads();  // s_waves is reused
     r = block_reduce<M>(r, s_waves);
     if (threadIdx.x == 0) {
-      if (p.mode == final_with_init) {
+      if (p.mode == final_with_init || p.mode == final_given_identity) {
         r = Op::apply(p.init, r);
       } else if (p.mode == final_accumulate) {
         r = Op::apply(static_cast<W>(*p.out), r);

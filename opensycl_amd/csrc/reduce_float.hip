@@ -48,4 +48,11 @@ acpp_status acpp_transform_reduce_dot_compensated_bf16(acpp_allocation_group *sc
   return compensated_entry<bf16_t>(scratch, static_cast<hipStream_t>(stream), reinterpret_cast<const bf16_t *>(first1), reinterpret_cast<const bf16_t *>(first2), true, n, out, init, config, "acpp_transform_reduce_dot_compensated_bf16");
 }
 
+acpp_status acpp_reduction_identity_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *out, float identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config) {
+  return reduce_entry<float, float>(scratch, static_cast<hipStream_t>(stream), first, n, out, identity, op, initialize_to_identity ? final_given_identity : final_accumulate, config, "acpp_reduction_identity_f32");
+}
+acpp_status acpp_reduction_identity_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *out, double identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config) {
+  return reduce_entry<double, double>(scratch, static_cast<hipStream_t>(stream), first, n, out, identity, op, initialize_to_identity ? final_given_identity : final_accumulate, config, "acpp_reduction_identity_f64");
+}
+
 }  // extern "C"

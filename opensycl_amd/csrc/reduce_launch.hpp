@@ -14,11 +14,13 @@ constexpr int vec_width() {
 }
 
 // XF: unary transform_reduce with element transform `unary` (unary.hpp).
-template <class T, class M, class Op, class OutT, bool DOT, bool XF = false>
+// BX (with DOT): binary transform_reduce with pair transform `binary`
+// (binary.hpp), combined with Op.
+template <class T, class M, class Op, class OutT, bool DOT, bool XF = false, bool BX = false>
 acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
                    const T *b, uint64_t n, OutT *out,
                    typename elem_traits<T>::work init, int mode,
-                   const acpp_launch_config *cfg, const char *name, int unary = 0) {
+                   const acpp_launch_config *cfg, const char *name, int unary = 0, int binary = 0) {
   using V = typename M::value;
   constexpr int VEC = vec_width<T>();
   int cus = 0;
@@ -39,7 +41,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   // in flight: their per-load Dot2 state makes 8 loads need 128 VGPRs and
   // spill, and 2 loads (45 VGPRs) measured fastest (2 x 2^28 fp32: 318 us
   // against 351 us with 8; bf16: 165 us against 177 us).
-  constexpr bool tuned = (Op::id == ops::op_plus) && !XF;
+  constexpr bool tuned = (Op::id == ops::op_plus) && !XF && !BX;
   constexpr bool comp_dot = DOT && std::is_same_v<M, comp_monoid>;
   constexpr bool narrow8 = sizeof(typename elem_traits<T>::storage) == 1;
   const bool cfg_given = cfg && (cfg->work_group_size || cfg->items_per_thread || cfg->groups_per_cu ||
@@ -92,6 +94,8 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   p.init = init;
   p.mode = mode;
   p.unary = unary;
+  p.binary = binary;
+  p.force_give_up = (cfg && (cfg->flags & ACPP_VARIANT_FORCE_GIVE_UP)) ? 1 : 0;
 
   auto run = [&](auto kernel) -> acpp_status {
     int resident = 1;
@@ -106,17 +110,25 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
     uint64_t grid = needed < 1 ? 1 : needed;
     if (grid > max_groups) grid = max_groups;
 
-    void *partials = nullptr;
-    s = group_obtain(g, grid * sizeof(V), &partials);
-    if (s != ACPP_SUCCESS) return s;
-    void *sync = nullptr;
+    // One sync region: the header (done ticket) and a tagged slot per
+    // work-group for its partial.
+    static_assert(sizeof(V) <= partial_slot_words * 4, "partial does not fit its slot");
+    const size_t used = sizeof(sync_header) + grid * partial_slot_words * sizeof(uint64_t);
+    void *region = nullptr;
     size_t cap = 0;
-    s = group_obtain_sync(g, sizeof(sync_header), stream, &sync, &cap);
+    region_state *rs = nullptr;
+    s = group_obtain_sync(g, used, stream, &region, &cap, &rs);
     if (s != ACPP_SUCCESS) return s;
-    p.partials = static_cast<V *>(partials);
-    p.sync = static_cast<sync_header *>(sync);
+    region_launch rl;
+    s = begin_region_launch(g, region, used, cap, rs, stream, &rl);
+    if (s != ACPP_SUCCESS) return s;
+    p.sync = static_cast<sync_header *>(region);
+    p.partials = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
+    p.epoch = rl.epoch;
     hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(grid)), dim3(wg), 0, stream, p);
-    return check_launch(name);
+    s = check_launch(name);
+    end_region_launch(rs, rl, 0, s);
+    return s;
   };
 
   // Only sums (the tuned hot path) get every unroll depth and the
@@ -136,8 +148,8 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *a,
   // Other operators and the transform forms: 4 loads in flight,
   // non-temporal above 256 MiB as for sums (2^28 int32 max: 170 us with
   // ordinary loads).
-  if (nt) return run(reduce_kernel<T, M, Op, OutT, 4, DOT, true, XF>);
-  return run(reduce_kernel<T, M, Op, OutT, 4, DOT, false, XF>);
+  if (nt) return run(reduce_kernel<T, M, Op, OutT, 4, DOT, true, XF, BX>);
+  return run(reduce_kernel<T, M, Op, OutT, 4, DOT, false, XF, BX>);
 }
 
 // algorithms::reduce / sycl::reduction entry for element type T.
@@ -225,6 +237,44 @@ acpp_status transform_reduce_entry(acpp_allocation_group *g, hipStream_t stream,
                                                              elem_traits<T>::to_work(init),
                                                              final_with_init, cfg, name, unary);
     return s == ACPP_SUCCESS ? clear_error() : s;
+  });
+}
+
+// Binary transform_reduce (numeric.hpp:170-191) with reduction operator
+// `op` (plus, multiplies, minimum or maximum) and pair transform `binary`
+// (binary.hpp): *out = op(init, op over binary(a[i], b[i])); an empty input
+// leaves *out untouched (numeric.hpp:165-169).  plus with multiplies is
+// the dot product's own kernel.
+template <class T>
+acpp_status binary_transform_reduce_entry(acpp_allocation_group *g, hipStream_t stream, const T *a, uint64_t n,
+                                          const T *b, T *out, T init, int op, int binary,
+                                          const acpp_launch_config *cfg, const char *name) {
+  using W = typename elem_traits<T>::work;
+  if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
+  if (!out) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": out is NULL");
+  if (n > 0 && (!a || !b)) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": input is NULL");
+  if (!binary_detail::valid_code<T>(binary))
+    return set_error(std::is_integral_v<T> || binary < 0 || binary >= binary_detail::b_count
+                         ? ACPP_ERROR_INVALID_ARGUMENT
+                         : ACPP_ERROR_UNSUPPORTED,
+                     std::string(name) + ": unknown pair transform for this element type");
+  if (op < ops::op_plus || op > ops::op_logical_or)
+    return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": unknown operator");
+  if (op > ops::op_max)
+    return set_error(ACPP_ERROR_UNSUPPORTED,
+                     std::string(name) + ": the binary form reduces with plus, multiplies, minimum or maximum");
+  if (n == 0) return clear_error();
+  if (op == ops::op_plus && binary == binary_detail::b_multiplies)
+    return dot_entry<T, T>(g, stream, a, n, b, out, elem_traits<T>::to_work(init), cfg, name);
+  return with_op<W, T>(op, [&](auto opv) -> acpp_status {
+    using Op = decltype(opv);
+    if constexpr (Op::id > ops::op_max) {
+      return set_error(ACPP_ERROR_UNSUPPORTED, std::string(name) + ": unsupported operator");
+    } else {
+      acpp_status s = launch<T, plain_monoid<W, Op>, Op, T, true, false, true>(
+          g, stream, a, b, n, out, elem_traits<T>::to_work(init), final_with_init, cfg, name, 0, binary);
+      return s == ACPP_SUCCESS ? clear_error() : s;
+    }
   });
 }
 

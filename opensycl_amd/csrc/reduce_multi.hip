@@ -19,9 +19,10 @@ namespace multi_detail {
 
 using reduce_detail::block_reduce;
 using reduce_detail::max_waves;
-using reduce_detail::load_partial;
+using reduce_detail::partial_slot_words;
 using reduce_detail::plain_monoid;
-using reduce_detail::store_partial;
+using reduce_detail::publish_partial;
+using reduce_detail::wait_partial;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int max_inputs = 4;
@@ -35,9 +36,15 @@ struct params {
   int op[max_inputs];
   int count;
   uint64_t n;
-  int identity_init;  // 1: out = fold; 0: out = op(out, fold)
-  W *partials;        // [count][gridDim.x]
+  // Per input: 0 out = op(out, fold); 1 out = fold (initialize_to_identity);
+  // 2 out = op(identity[k], fold) (initialize_to_identity with a given
+  // identity, which an empty range yields).
+  int init_mode[max_inputs];
+  W identity[max_inputs];
+  uint64_t *partials; // tagged slots [count][gridDim.x] (reduce_common.hpp)
   sync_header *sync;
+  uint32_t epoch;     // tag of this launch's partials (>= 1)
+  int force_give_up;  // ACPP_VARIANT_FORCE_GIVE_UP (tests of the error path)
 };
 
 // Calls f(Op{}) with the operator functor for `code`.
@@ -118,11 +125,14 @@ __global__ __launch_bounds__(1024) void multi_kernel(params<T> p) {
       using Op = decltype(opv);
       W v = stream_fold<T, Op, NT>(p.in[k], p.n);
       v = block_reduce<plain_monoid<W, Op>>(v, s_waves);
-      if (threadIdx.x == 0) store_partial(p.partials + static_cast<uint64_t>(k) * gridDim.x + blockIdx.x, v);
+      if (threadIdx.x == 0)
+        publish_partial(p.partials, static_cast<uint64_t>(k) * gridDim.x + blockIdx.x, p.epoch, v);
     });
   }
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_s_waitcnt(0);  // partials stored before the ticket
+    // The last group waits for each partial's tag; waiting for the stores
+    // first only makes such a wait rare.
+    wait_own_stores();
     const uint32_t prev =
         __hip_atomic_fetch_add(&p.sync->done_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (prev == gridDim.x - 1);
@@ -135,10 +145,12 @@ __global__ __launch_bounds__(1024) void multi_kernel(params<T> p) {
       using Op = decltype(opv);
       W r = Op::identity();
       for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-        r = Op::apply(r, load_partial(p.partials + static_cast<uint64_t>(k) * gridDim.x + i));
+        r = Op::apply(r, wait_partial<W>(p.partials, static_cast<uint64_t>(k) * gridDim.x + i, p.epoch, p.sync,
+                                         p.force_give_up != 0, Op::identity()));
       r = block_reduce<plain_monoid<W, Op>>(r, s_waves);
       if (threadIdx.x == 0) {
-        if (!p.identity_init) r = Op::apply(elem_traits<T>::to_work(*p.out[k]), r);
+        if (p.init_mode[k] == 0) r = Op::apply(elem_traits<T>::to_work(*p.out[k]), r);
+        else if (p.init_mode[k] == 2) r = Op::apply(p.identity[k], r);
         *p.out[k] = static_cast<T>(r);
       }
     });
@@ -147,29 +159,39 @@ __global__ __launch_bounds__(1024) void multi_kernel(params<T> p) {
     __hip_atomic_store(&p.sync->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// n == 0 with initialize_to_identity: every output becomes its identity.
+// n == 0: outputs with initialize_to_identity become their identity (the
+// given one, or the operator's); the others keep their value.
 template <class T>
 __global__ void identity_kernel(params<T> p) {
   using W = typename elem_traits<T>::work;
   const int k = threadIdx.x;
-  if (k < p.count)
-    dispatch<W>(p.op[k], [&](auto opv) { *p.out[k] = static_cast<T>(decltype(opv)::identity()); });
+  if (k < p.count && p.init_mode[k] != 0)
+    dispatch<W>(p.op[k], [&](auto opv) {
+      *p.out[k] = static_cast<T>(p.init_mode[k] == 2 ? p.identity[k] : decltype(opv)::identity());
+    });
 }
 
+// identities: NULL, or one value per input (used where the input's
+// initialize_to_identity flag is set).  init_flags: one flag per input.
 template <class T>
 acpp_status entry(acpp_allocation_group *g, hipStream_t stream, uint32_t count, const T *const *inputs,
-                  uint64_t n, T *const *outs, const acpp_algorithm_op *op_codes, int identity_init,
-                  const acpp_launch_config *cfg, const char *name) {
+                  uint64_t n, T *const *outs, const acpp_algorithm_op *op_codes, const T *identities,
+                  const int *init_flags, const acpp_launch_config *cfg, const char *name) {
   using W = typename elem_traits<T>::work;
   if (!g) return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": scratch group is NULL");
   if (count < 1 || count > static_cast<uint32_t>(max_inputs))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": count must be 1..4");
-  if (!inputs || !outs || !op_codes)
+  if (!inputs || !outs || !op_codes || !init_flags)
     return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": NULL array argument");
   params<T> p{};
   p.count = static_cast<int>(count);
   p.n = n;
-  p.identity_init = identity_init ? 1 : 0;
+  bool any_init = false;
+  for (uint32_t k = 0; k < count; ++k) {
+    p.init_mode[k] = init_flags[k] ? (identities ? 2 : 1) : 0;
+    p.identity[k] = identities ? elem_traits<T>::to_work(identities[k]) : W{};
+    any_init = any_init || init_flags[k];
+  }
   for (uint32_t k = 0; k < count; ++k) {
     const int op = op_codes[k];
     if (op < ops::op_plus || op > ops::op_logical_or)
@@ -189,7 +211,7 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, uint32_t count, 
   if (n == 0) {
     // reduction_descriptor.hpp:36-53: with initialize_to_identity the output
     // is the identity; without it the output keeps its value.
-    if (p.identity_init) {
+    if (any_init) {
       hipLaunchKernelGGL(identity_kernel<T>, dim3(1), dim3(64), 0, stream, p);
       st = check_launch(name);
       if (st != ACPP_SUCCESS) return st;
@@ -207,15 +229,19 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, uint32_t count, 
   constexpr int VEC = 16 / sizeof(T);
   const uint64_t needed = ceil_div(ceil_div(n, static_cast<uint64_t>(VEC)), static_cast<uint64_t>(wg) * unroll);
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(cap, needed)));
-  void *partials = nullptr;
-  st = group_obtain(g, static_cast<size_t>(grid) * count * sizeof(W), &partials);
-  if (st != ACPP_SUCCESS) return st;
-  void *sync = nullptr;
+  const size_t used = sizeof(sync_header) + static_cast<size_t>(grid) * count * partial_slot_words * sizeof(uint64_t);
+  void *region = nullptr;
   size_t cap_bytes = 0;
-  st = group_obtain_sync(g, sizeof(sync_header), stream, &sync, &cap_bytes);
+  region_state *rs = nullptr;
+  st = group_obtain_sync(g, used, stream, &region, &cap_bytes, &rs);
   if (st != ACPP_SUCCESS) return st;
-  p.partials = static_cast<W *>(partials);
-  p.sync = static_cast<sync_header *>(sync);
+  region_launch rl;
+  st = begin_region_launch(g, region, used, cap_bytes, rs, stream, &rl);
+  if (st != ACPP_SUCCESS) return st;
+  p.sync = static_cast<sync_header *>(region);
+  p.partials = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
+  p.epoch = rl.epoch;
+  p.force_give_up = (cfg && (cfg->flags & ACPP_VARIANT_FORCE_GIVE_UP)) ? 1 : 0;
   // Non-temporal loads once the inputs are larger than the 256 MiB
   // last-level cache, as for the plain reductions.
   const bool nt = static_cast<uint64_t>(count) * n * sizeof(T) >= (256ull << 20);
@@ -224,6 +250,7 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, uint32_t count, 
   else
     hipLaunchKernelGGL((multi_kernel<T, false>), dim3(grid), dim3(wg), 0, stream, p);
   st = check_launch(name);
+  end_region_launch(rs, rl, 0, st);
   return st == ACPP_SUCCESS ? clear_error() : st;
 }
 
@@ -239,8 +266,18 @@ extern "C" {
                                          uint32_t count, const T *const *inputs, uint64_t n,           \
                                          T *const *outs, const acpp_algorithm_op *ops,                 \
                                          int initialize_to_identity, const acpp_launch_config *config) { \
+    const int flags[multi_detail::max_inputs] = {initialize_to_identity, initialize_to_identity,       \
+                                                 initialize_to_identity, initialize_to_identity};      \
     return multi_detail::entry<T>(scratch, static_cast<hipStream_t>(stream), count, inputs, n, outs, ops, \
-                                  initialize_to_identity, config, "acpp_reduction_multi_" #tag);       \
+                                  nullptr, flags, config, "acpp_reduction_multi_" #tag);               \
+  }                                                                                                     \
+  acpp_status acpp_reduction_multi_identity_##tag(                                                     \
+      acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const T *const *inputs,    \
+      uint64_t n, T *const *outs, const acpp_algorithm_op *ops, const T *identities,                   \
+      const int *initialize_to_identity, const acpp_launch_config *config) {                           \
+    return multi_detail::entry<T>(scratch, static_cast<hipStream_t>(stream), count, inputs, n, outs, ops, \
+                                  identities, initialize_to_identity, config,                          \
+                                  "acpp_reduction_multi_identity_" #tag);                             \
   }
 ACPP_DEFINE_MULTI(i32, int32_t)
 ACPP_DEFINE_MULTI(u32, uint32_t)

@@ -43,4 +43,17 @@ acpp_status acpp_transform_reduce_dot_i64(acpp_allocation_group *scratch, acpp_s
   return dot_entry<int64_t, int64_t>(scratch, static_cast<hipStream_t>(stream), first1, n, first2, out, init, config, "acpp_transform_reduce_dot_i64");
 }
 
+acpp_status acpp_reduction_identity_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *out, int8_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config) {
+  return reduce_entry<int8_t, int8_t>(scratch, static_cast<hipStream_t>(stream), first, n, out, identity, op, initialize_to_identity ? final_given_identity : final_accumulate, config, "acpp_reduction_identity_i8");
+}
+acpp_status acpp_reduction_identity_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *out, int16_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config) {
+  return reduce_entry<int16_t, int16_t>(scratch, static_cast<hipStream_t>(stream), first, n, out, identity, op, initialize_to_identity ? final_given_identity : final_accumulate, config, "acpp_reduction_identity_i16");
+}
+acpp_status acpp_reduction_identity_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *out, int32_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config) {
+  return reduce_entry<int32_t, int32_t>(scratch, static_cast<hipStream_t>(stream), first, n, out, identity, op, initialize_to_identity ? final_given_identity : final_accumulate, config, "acpp_reduction_identity_i32");
+}
+acpp_status acpp_reduction_identity_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *out, int64_t identity, acpp_algorithm_op op, int initialize_to_identity, const acpp_launch_config *config) {
+  return reduce_entry<int64_t, int64_t>(scratch, static_cast<hipStream_t>(stream), first, n, out, identity, op, initialize_to_identity ? final_given_identity : final_accumulate, config, "acpp_reduction_identity_i64");
+}
+
 }  // extern "C"

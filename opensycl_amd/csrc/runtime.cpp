@@ -9,6 +9,7 @@
 // is what lets the scan and the reduction run as single launches.
 #include <algorithm>
 #include <cstddef>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -66,6 +67,63 @@ acpp_status query_compute_units(int dev, int *out) {
   }
   *out = g_devices[dev].compute_units;
   return ACPP_SUCCESS;
+}
+// One pinned host word per device, raised by report_trip on the device.
+// It is allocated once per process (mapped into every device's address
+// space) and never freed.
+constexpr int max_devices = 64;
+std::mutex g_trip_mutex;
+uint32_t *g_trip_flags = nullptr;
+}  // namespace
+
+uint32_t *host_trip_flag(int device) {
+  if (device < 0 || device >= max_devices) return nullptr;
+  std::lock_guard<std::mutex> lock{g_trip_mutex};
+  if (!g_trip_flags) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, max_devices * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocPortable) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;  // kernels then only count trips in their region
+    }
+    memset(p, 0, max_devices * sizeof(uint32_t));
+    g_trip_flags = static_cast<uint32_t *>(p);
+  }
+  return g_trip_flags + device;
+}
+
+uint32_t *host_trip_flag_if_allocated(int device) {
+  if (device < 0 || device >= max_devices) return nullptr;
+  std::lock_guard<std::mutex> lock{g_trip_mutex};
+  return g_trip_flags ? g_trip_flags + device : nullptr;
+}
+
+namespace {
+// Take (read and clear) the trip flag of `device`.
+bool take_trip_flag(int device) {
+  if (device < 0 || device >= max_devices) return false;
+  uint32_t *flags = nullptr;
+  {
+    std::lock_guard<std::mutex> lock{g_trip_mutex};
+    flags = g_trip_flags;
+  }
+  if (!flags) return false;
+  return __atomic_exchange_n(flags + device, 0u, __ATOMIC_ACQ_REL) != 0;
+}
+
+// Peek at the trip flag of `device` without clearing it.
+bool trip_flag_raised(int device) {
+  if (device < 0 || device >= max_devices) return false;
+  std::lock_guard<std::mutex> lock{g_trip_mutex};
+  return g_trip_flags && __atomic_load_n(g_trip_flags + device, __ATOMIC_ACQUIRE) != 0;
+}
+
+acpp_status report_trips(int device, const char *where, bool clear) {
+  if (!(clear ? take_trip_flag(device) : trip_flag_raised(device))) return ACPP_SUCCESS;
+  return set_error(ACPP_ERROR_WATCHDOG,
+                   std::string(where) + ": a kernel on device " + std::to_string(device) +
+                       " gave up waiting for another work-group (watchdog); results written "
+                       "since the last check may be wrong");
 }
 }  // namespace
 
@@ -187,7 +245,7 @@ acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
     if (s != ACPP_SUCCESS) return s;
     a.sync = true;
     a.state = std::make_shared<region_state>();
-    s = fill_zero(a.ptr, a.size, stream);
+    s = reset_sync_region(a.ptr, a.size, g->dev, stream);
     if (s != ACPP_SUCCESS) {
       g->parent->deallocate(a);
       return s;
@@ -198,6 +256,43 @@ acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
   *capacity_bytes = a.size;
   if (state) *state = a.state.get();
   return ACPP_SUCCESS;
+}
+
+acpp_status begin_region_launch(acpp_allocation_group *g, void *region, size_t used, size_t capacity,
+                                region_state *rs, hipStream_t stream, region_launch *out) {
+  return begin_region_launch_flag(host_trip_flag(g->dev), region, used, capacity, rs, stream, out);
+}
+
+acpp_status begin_region_launch_flag(uint32_t *trip_flag, void *region, size_t used, size_t capacity,
+                                     region_state *rs, hipStream_t stream, region_launch *out) {
+  region_launch l;
+  l.capturing = stream_is_capturing(stream);
+  if (l.capturing) {
+    acpp_status st = reset_sync_region_flag(region, used, trip_flag, stream);
+    if (st != ACPP_SUCCESS) return st;
+    rs->mode = region_state::graph;
+  } else {
+    if (rs->mode == region_state::graph || rs->epoch >= max_epoch) {
+      acpp_status st = reset_sync_region_flag(region, capacity, trip_flag, stream);
+      if (st != ACPP_SUCCESS) return st;
+      rs->epoch = 0;
+      rs->ticket_base = 0;
+    }
+    rs->mode = region_state::host_managed;
+    l.epoch = rs->epoch + 1;
+    l.ticket_base = rs->ticket_base;
+  }
+  *out = l;
+  return ACPP_SUCCESS;
+}
+
+void end_region_launch(region_state *rs, const region_launch &l, uint32_t tickets, acpp_status launch_status) {
+  if (launch_status != ACPP_SUCCESS) {
+    rs->mode = region_state::graph;
+  } else if (!l.capturing) {
+    rs->epoch = l.epoch;
+    rs->ticket_base = l.ticket_base + tickets;
+  }
 }
 
 int group_device(const acpp_allocation_group *g) { return g->dev; }
@@ -278,14 +373,24 @@ acpp_status acpp_allocation_group_release(acpp_allocation_group *group) {
   if (!group) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "group is NULL");
   for (const allocation &a : group->held) group->parent->give_back(a);
   group->held.clear();
-  return clear_error();
+  acpp_status s = report_trips(group->dev, "acpp_allocation_group_release", false);
+  return s == ACPP_SUCCESS ? clear_error() : s;
 }
 
 acpp_status acpp_allocation_group_destroy(acpp_allocation_group *group) {
   if (!group) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "group is NULL");
-  acpp_allocation_group_release(group);
+  for (const allocation &a : group->held) group->parent->give_back(a);
+  group->held.clear();
+  const int dev = group->dev;
   delete group;
-  return clear_error();
+  acpp_status s = report_trips(dev, "acpp_allocation_group_destroy", false);
+  return s == ACPP_SUCCESS ? clear_error() : s;
+}
+
+acpp_status acpp_check_asynchronous_errors(int device) {
+  if (device < 0) return set_error(ACPP_ERROR_INVALID_ARGUMENT, "negative device id");
+  acpp_status s = report_trips(device, "acpp_check_asynchronous_errors", true);
+  return s == ACPP_SUCCESS ? clear_error() : s;
 }
 
 acpp_status acpp_allocation_group_watchdog_trips(acpp_allocation_group *group, uint64_t *trips) {

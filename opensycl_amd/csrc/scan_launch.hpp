@@ -18,15 +18,23 @@ constexpr int default_rows() {
 }
 
 // XF: the transform scans (element transform `unary`, see unary.hpp).
+// fixed: a caller-provided sync region (the no-allocation entry point) in
+// place of one from the group `g` (then unused).
 template <class T, class Op, bool XF = false>
 acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, uint64_t n,
                    T *out, bool exclusive, bool has_init, typename elem_traits<T>::work init,
-                   const acpp_launch_config *cfg, const char *name, int unary = 0) {
+                   const acpp_launch_config *cfg, const char *name, int unary = 0,
+                   const fixed_region *fixed = nullptr) {
   using W = typename elem_traits<T>::work;
   constexpr int VEC = 16 / sizeof(T);
   int cus = 0;
-  acpp_status st = enter_device(g, &cus);
-  if (st != ACPP_SUCCESS) return st;
+  acpp_status st = ACPP_SUCCESS;
+  if (fixed) {
+    cus = fixed->compute_units;
+  } else {
+    st = enter_device(g, &cus);
+    if (st != ACPP_SUCCESS) return st;
+  }
 
   // Default work-group size: 1024-wide tiles (fewest tiles, so the fewest
   // look-back links) when the input still gives every CU a few tiles,
@@ -59,34 +67,30 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   using SIO = status_io<W>;
   static_assert(SIO::words <= status_stride_words, "status slot too small");
   const uint64_t status_bytes = num_tiles * status_stride_words * sizeof(uint64_t);
+  const size_t used = sizeof(sync_header) + status_bytes;
   void *region = nullptr;
   size_t cap = 0;
   region_state *rs = nullptr;
-  st = group_obtain_sync(g, sizeof(sync_header) + status_bytes, stream, &region, &cap, &rs);
-  if (st != ACPP_SUCCESS) return st;
-
-  // Epoch and ticket base.  Eager launches continue the region's host-side
-  // count.  Under stream capture the graph may be replayed any number of
-  // times, so the captured work clears the region first (with a kernel) and
-  // always runs with epoch 1 and ticket base 0.
-  const bool capturing = stream_is_capturing(stream);
-  uint32_t epoch = 1, ticket_base = 0;
-  const size_t used = sizeof(sync_header) + status_bytes;
-  if (capturing) {
-    st = fill_zero(region, used, stream);
-    if (st != ACPP_SUCCESS) return st;
-    rs->mode = region_state::graph;
+  region_launch rl;
+  if (fixed) {
+    if (used > fixed->capacity)
+      return set_error(ACPP_ERROR_INVALID_ARGUMENT,
+                       std::string(name) + ": " + std::to_string(num_tiles) + " tiles need " +
+                           std::to_string(used) + " bytes of scratch; the static scratch holds " +
+                           std::to_string(fixed->capacity) + " (at most " +
+                           std::to_string((fixed->capacity - sizeof(sync_header)) /
+                                          (status_stride_words * sizeof(uint64_t))) +
+                           " tiles of " + std::to_string(tile_elems) + " elements)");
+    region = fixed->ptr;
+    cap = fixed->capacity;
+    rs = fixed->rs;
+    st = begin_region_launch_flag(fixed->trip_flag, region, used, cap, rs, stream, &rl);
   } else {
-    if (rs->mode == region_state::graph || rs->epoch >= max_epoch) {
-      st = fill_zero(region, cap, stream);
-      if (st != ACPP_SUCCESS) return st;
-      rs->epoch = 0;
-      rs->ticket_base = 0;
-    }
-    rs->mode = region_state::host_managed;
-    epoch = rs->epoch + 1;
-    ticket_base = rs->ticket_base;
+    st = group_obtain_sync(g, used, stream, &region, &cap, &rs);
+    if (st != ACPP_SUCCESS) return st;
+    st = begin_region_launch(g, region, used, cap, rs, stream, &rl);
   }
+  if (st != ACPP_SUCCESS) return st;
 
   params<T> p;
   p.in = in;
@@ -100,28 +104,24 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   p.vec_io = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) &
               (sizeof(T) >= 4 ? 3u : 15u)) == 0;
   p.block_lookback = (cfg && (cfg->flags & variant_block_lookback)) ? 1 : 0;
+  p.force_give_up = (cfg && (cfg->flags & ACPP_VARIANT_FORCE_GIVE_UP)) ? 1 : 0;
   p.unary = unary;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
-  p.epoch = epoch;
-  p.ticket_base = ticket_base;
+  p.epoch = rl.epoch;
+  p.ticket_base = rl.ticket_base;
   p.sync = static_cast<sync_header *>(region);
   p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
 
   auto run = [&](auto kernel) -> acpp_status {
     hipLaunchKernelGGL(kernel, dim3(static_cast<uint32_t>(num_tiles)), dim3(wg), 0, stream, p);
     acpp_status s = check_launch(name);
-    if (s == ACPP_SUCCESS && !capturing) {
-      rs->epoch = epoch;
-      rs->ticket_base = ticket_base + static_cast<uint32_t>(num_tiles);
-    } else if (s != ACPP_SUCCESS) {
-      // The launch may or may not have run: start the region over next time.
-      rs->mode = region_state::graph;
-    }
+    end_region_launch(rs, rl, static_cast<uint32_t>(num_tiles), s);
     return s;
   };
   if constexpr (tunable) {
     if (rows == 2) return run(scan_kernel<T, Op, 2>);
     if (rows == 4) return run(scan_kernel<T, Op, 4>);
+    if (cfg && (cfg->flags & ACPP_VARIANT_NONTEMPORAL)) return run(scan_kernel<T, Op, default_rows<T>(), false, true>);
   }
   return run(scan_kernel<T, Op, default_rows<T>(), XF>);
 }

@@ -180,7 +180,7 @@ __device__ void tile_pair_prefixes(int P, uint32_t *s_tf, W *s_tv, uint32_t &agg
 // there in tile order.  Must be called by a whole wave; result uniform.
 template <class Op, class W>
 __device__ void wave_pair_lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
-                                   uint32_t *watchdog, uint32_t &out_f, W &out_v) {
+                                   sync_header *sync, bool force_give_up, uint32_t &out_f, W &out_v) {
   using SIO = seg_status<W>;
   const int lane = threadIdx.x & 63;
   const W id = Op::identity();
@@ -189,7 +189,8 @@ __device__ void wave_pair_lookback(const uint64_t *status, uint32_t tile, uint32
   W ev = id;
   lookback_wait wait;
   wait.begin();
-  bool give_up = false;  // uniform
+  bool give_up = force_give_up;  // uniform
+  if (give_up && lane == 0) report_trip(sync);
   for (;;) {
     const int64_t j = wbase - (63 - lane);
     uint32_t lf = 0;
@@ -209,7 +210,7 @@ __device__ void wave_pair_lookback(const uint64_t *status, uint32_t tile, uint32
       if (bad == 0) break;
       if (wait.failed_round()) {
         give_up = true;
-        if (lane == 0) __hip_atomic_fetch_add(watchdog, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) report_trip(sync);
       }
     }
     uint32_t xf = (lane >= q) ? lf : 0u;
@@ -236,6 +237,7 @@ struct params {
   const uint32_t *heads;     // bit e set: a non-empty segment starts at element e
   uint64_t head_words_total;  // 32-bit words of the bitmap
   int block_lookback;        // 1: work-group-wide look-back window
+  int force_give_up;         // ACPP_VARIANT_FORCE_GIVE_UP (tests of the error path)
   uint64_t num_segments;
   uint64_t total;
   W init;
@@ -454,7 +456,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
           uint32_t ef;
           W ev;
           wait_own_stores();  // the aggregate is out before the first poll
-          wave_pair_lookback<Op, W>(p.status, tile, p.epoch, &p.sync->watchdog_trips, ef, ev);
+          wave_pair_lookback<Op, W>(p.status, tile, p.epoch, p.sync, p.force_give_up != 0, ef, ev);
           if (lane == 0) {
             s_lbf[0] = ef;
             s_lbv[0] = ev;
@@ -476,7 +478,8 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
         W ev = id;
         lookback_wait wait;
         wait.begin();
-        bool give_up = false;  // uniform across the work-group
+        bool give_up = p.force_give_up != 0;  // uniform across the work-group
+        if (give_up && tid == 0) report_trip(p.sync);
         for (;;) {
           const int64_t j = wbase - (B - 1 - tid);
           uint32_t lf = 0;
@@ -517,8 +520,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
             const bool late = wait.failed_round();
             if (tid == 0) {
               s_give_up = late ? 1 : 0;
-              if (late)
-                __hip_atomic_fetch_add(&p.sync->watchdog_trips, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (late) report_trip(p.sync);
             }
             __syncthreads();
             give_up = s_give_up != 0;
@@ -892,6 +894,7 @@ acpp_status enqueue_reduce(acpp_allocation_group *g, launch_plan<T, Op> &lp, hip
   p.heads = lp.heads;
   p.head_words_total = lp.head_words;
   p.block_lookback = 0;
+  p.force_give_up = 0;
   p.num_segments = num_segments;
   p.total = total;
   p.init = init;
@@ -970,23 +973,9 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   st = group_obtain_sync(g, lp.region_used, stream, &lp.region, &lp.region_cap, &lp.rs);
   if (st != ACPP_SUCCESS) return st;
 
-  const bool capturing = stream_is_capturing(stream);
-  uint32_t epoch = 1, ticket_base = 0;
-  if (capturing) {
-    st = fill_zero(lp.region, lp.region_used, stream);
-    if (st != ACPP_SUCCESS) return st;
-    lp.rs->mode = region_state::graph;
-  } else {
-    if (lp.rs->mode == region_state::graph || lp.rs->epoch >= max_epoch) {
-      st = fill_zero(lp.region, lp.region_cap, stream);
-      if (st != ACPP_SUCCESS) return st;
-      lp.rs->epoch = 0;
-      lp.rs->ticket_base = 0;
-    }
-    lp.rs->mode = region_state::host_managed;
-    epoch = lp.rs->epoch + 1;
-    ticket_base = lp.rs->ticket_base;
-  }
+  region_launch rl;
+  st = begin_region_launch(g, lp.region, lp.region_used, lp.region_cap, lp.rs, stream, &rl);
+  if (st != ACPP_SUCCESS) return st;
 
   params<T> &p = lp.p;
   p.in = in;
@@ -996,14 +985,15 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   p.heads = lp.heads;
   p.head_words_total = lp.head_words;
   p.block_lookback = (cfg && (cfg->flags & ACPP_VARIANT_BLOCK_LOOKBACK)) ? 1 : 0;
+  p.force_give_up = (cfg && (cfg->flags & ACPP_VARIANT_FORCE_GIVE_UP)) ? 1 : 0;
   p.num_segments = num_segments;
   p.total = total;
   p.init = init;
   p.reduce = reduce ? 1 : 0;
   p.vec_io = ((reinterpret_cast<uintptr_t>(in) | (reduce ? 0 : reinterpret_cast<uintptr_t>(out))) & 3) == 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
-  p.epoch = epoch;
-  p.ticket_base = ticket_base;
+  p.epoch = rl.epoch;
+  p.ticket_base = rl.ticket_base;
   p.sync = static_cast<sync_header *>(lp.region);
   p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(lp.region) + sizeof(sync_header));
 
@@ -1038,13 +1028,8 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
     if ((s = pick(segmented_kernel<T, Op, 8>)) != ACPP_SUCCESS) return s;
     s = launch_rows<T, Op, 8>(lp, stream, name);
   }
-  if (s == ACPP_SUCCESS && !capturing) {
-    lp.rs->epoch = epoch;
-    // every work-group draws one ticket past the last tile before it exits
-    lp.rs->ticket_base = ticket_base + static_cast<uint32_t>(num_tiles) + lp.grid;
-  } else if (s != ACPP_SUCCESS) {
-    lp.rs->mode = region_state::graph;
-  }
+  // every work-group draws one ticket past the last tile before it exits
+  end_region_launch(lp.rs, rl, static_cast<uint32_t>(num_tiles) + lp.grid, s);
   return s;
 }
 

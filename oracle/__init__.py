@@ -326,3 +326,20 @@ def transform_inclusive_scan(x: np.ndarray, op: int, code: int, init=None) -> np
 def transform_exclusive_scan(x: np.ndarray, init, op: int, code: int) -> np.ndarray:
     """transform_exclusive_scan (numeric.hpp:319-327; scan.hpp:85-132)."""
     return exclusive_scan(transform(x, code), init, op)
+
+
+# pair transform codes (== acpp_binary_op)
+B_MULTIPLIES, B_PLUS, B_MINUS, B_ABS_DIFF, B_SQ_DIFF, B_MIN, B_MAX, B_BIT_AND, B_BIT_OR, B_BIT_XOR = range(10)
+
+
+def transform_reduce_binary(a: np.ndarray, b: np.ndarray, init, op: int, code: int):
+    """Binary transform_reduce (numeric.hpp:170-191): serial fold of
+    op over t(a[i], b[i]) starting from init."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    t = _tag(a)
+    out = _CT[t]()
+    _ok(getattr(load(), f"oracle_transform_reduce_binary_{t}")(
+        c_void_p(_p(a)), c_void_p(_p(b)), c_uint64(a.size), _CT[t](init), c_int(op), c_int(code),
+        ctypes.byref(out)))
+    return a.dtype.type(out.value)

@@ -720,3 +720,76 @@ DEF_TRANSFORM_INT(u32, uint32_t, uint32_t, 0)
 DEF_TRANSFORM_INT(u64, uint64_t, uint64_t, 0)
 DEF_TRANSFORM_FP(f32, float, fabsf)
 DEF_TRANSFORM_FP(f64, double, fabs)
+
+/* ------------------------------------------------------------------------- */
+/* Binary transform_reduce (numeric.hpp:170-191): acc = init; acc =           */
+/* op(acc, t(a[i], b[i])) in order (std::transform_reduce's serial order, as  */
+/* tests/pstl/transform_reduce.cpp checks it).  Pair transform codes: 0 x*y,  */
+/* 1 x+y, 2 x-y, 3 |x-y|, 4 (x-y)^2, 5 x<y?x:y, 6 x>y?x:y, 7 x&y, 8 x|y,       */
+/* 9 x^y (integers).  Integer results wrap (computed in the unsigned WU).     */
+/* Fault 15 turns |x-y| into x-y.                                             */
+/* ------------------------------------------------------------------------- */
+#define DEF_BINARY_INT(t, T, WU, SIGNED)                                                        \
+  static int bt_##t(int code, T x, T y, T *r) {                                                 \
+    const WU a = (WU)x, b = (WU)y;                                                              \
+    switch (code) {                                                                             \
+      case 0: *r = (T)(a * b); return 0;                                                        \
+      case 1: *r = (T)(a + b); return 0;                                                        \
+      case 2: *r = (T)(a - b); return 0;                                                        \
+      case 3: {                                                                                 \
+        const T d = (T)(a - b);                                                                 \
+        const int neg = SIGNED && ((((WU)d) >> (8 * sizeof(T) - 1)) & 1u);                      \
+        *r = (neg && g_fault != 15) ? (T)((WU)0 - (WU)d) : d;                                   \
+        return 0;                                                                               \
+      }                                                                                         \
+      case 4: *r = (T)((a - b) * (a - b)); return 0;                                            \
+      case 5: *r = (x < y) ? x : y; return 0;                                                   \
+      case 6: *r = (x > y) ? x : y; return 0;                                                   \
+      case 7: *r = (T)(a & b); return 0;                                                        \
+      case 8: *r = (T)(a | b); return 0;                                                        \
+      case 9: *r = (T)(a ^ b); return 0;                                                        \
+      default: return -1;                                                                       \
+    }                                                                                           \
+  }
+#define DEF_BINARY_FP(t, T, FABS)                                                               \
+  static int bt_##t(int code, T x, T y, T *r) {                                                 \
+    switch (code) {                                                                             \
+      case 0: *r = x * y; return 0;                                                             \
+      case 1: *r = x + y; return 0;                                                             \
+      case 2: *r = x - y; return 0;                                                             \
+      case 3: *r = (g_fault == 15) ? (x - y) : FABS(x - y); return 0;                           \
+      case 4: {                                                                                 \
+        const T d = x - y;                                                                      \
+        *r = d * d;                                                                             \
+        return 0;                                                                               \
+      }                                                                                         \
+      case 5: *r = (x < y) ? x : y; return 0;                                                   \
+      case 6: *r = (x > y) ? x : y; return 0;                                                   \
+      default: return -1;                                                                       \
+    }                                                                                           \
+  }
+#define DEF_BINARY_REDUCE(t, T)                                                                 \
+  int oracle_transform_reduce_binary_##t(const T *a, const T *b, uint64_t n, T init, int op,    \
+                                         int code, T *out) {                                    \
+    if (!valid_op_##t(op)) return -1;                                                           \
+    T acc = init, v;                                                                            \
+    if (bt_##t(code, (T)0, (T)0, &v) != 0) return -1;                                           \
+    for (uint64_t i = 0; i < n; ++i) {                                                          \
+      bt_##t(code, a[i], b[i], &v);                                                             \
+      acc = ap_##t(op, acc, v);                                                                 \
+    }                                                                                           \
+    *out = acc;                                                                                 \
+    return 0;                                                                                   \
+  }
+DEF_BINARY_INT(i32, int32_t, uint32_t, 1)
+DEF_BINARY_INT(i64, int64_t, uint64_t, 1)
+DEF_BINARY_INT(u32, uint32_t, uint32_t, 0)
+DEF_BINARY_INT(u64, uint64_t, uint64_t, 0)
+DEF_BINARY_FP(f32, float, fabsf)
+DEF_BINARY_FP(f64, double, fabs)
+DEF_BINARY_REDUCE(i32, int32_t)
+DEF_BINARY_REDUCE(i64, int64_t)
+DEF_BINARY_REDUCE(u32, uint32_t)
+DEF_BINARY_REDUCE(u64, uint64_t)
+DEF_BINARY_REDUCE(f32, float)
+DEF_BINARY_REDUCE(f64, double)

@@ -102,7 +102,8 @@ int oracle_num_procs(void);
    histogram key, 6 short copy_if count, 7 abs(x) = x; in the OpenMP-order
    restatements: 8 sixteen elements per work-item, 9 serial left fold, 10
    adjacent-pairs tree, 11 swapped tree operands, 12 1024 x 8 scan tiles, 13
-   swapped work-item fold operands, 14 op(result, init). */
+   swapped work-item fold operands, 14 op(result, init); 15 |x-y| computed
+   as x-y in the binary transform_reduce. */
 void oracle_set_fault(int fault);
 
 /* Element transforms (transform codes 0..7 of acpp_unary_op); -1 for an
@@ -117,6 +118,16 @@ int oracle_transform_u32(const uint32_t *x, uint64_t n, uint32_t *y, int code);
 int oracle_transform_u64(const uint64_t *x, uint64_t n, uint64_t *y, int code);
 int oracle_transform_f32(const float *x, uint64_t n, float *y, int code);
 int oracle_transform_f64(const double *x, uint64_t n, double *y, int code);
+
+/* Binary transform_reduce: *out = op(...op(op(init, t(a0, b0)), t(a1, b1))...)
+   for pair transform codes 0..9 (acpp_binary_op); -1 for an unknown code or
+   operator, or a bitwise code on floating point. */
+int oracle_transform_reduce_binary_i32(const int32_t *a, const int32_t *b, uint64_t n, int32_t init, int op, int code, int32_t *out);
+int oracle_transform_reduce_binary_i64(const int64_t *a, const int64_t *b, uint64_t n, int64_t init, int op, int code, int64_t *out);
+int oracle_transform_reduce_binary_u32(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t init, int op, int code, uint32_t *out);
+int oracle_transform_reduce_binary_u64(const uint64_t *a, const uint64_t *b, uint64_t n, uint64_t init, int op, int code, uint64_t *out);
+int oracle_transform_reduce_binary_f32(const float *a, const float *b, uint64_t n, float init, int op, int code, float *out);
+int oracle_transform_reduce_binary_f64(const double *a, const double *b, uint64_t n, double init, int op, int code, double *out);
 
 #ifdef __cplusplus
 }

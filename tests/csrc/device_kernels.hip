@@ -209,6 +209,82 @@ __global__ void atomic_fold_kernel(int op, T *data, uint64_t n) {
   }
 }
 
+// 8- and 16-bit atomic builtins: work-items slots..n-1 apply the op with
+// operand data[i] to data[i % slots].  With slots < 4 bytes' worth of
+// elements, neighbouring elements of one 32-bit word are updated
+// concurrently.  rmw op numbering as above.
+#define TK_NARROW_SWITCH(tag)                                                      \
+  switch (op) {                                                                    \
+    case 0: __acpp_sscp_atomic_fetch_and_##tag(as, o, s, t, x); break;             \
+    case 1: __acpp_sscp_atomic_fetch_or_##tag(as, o, s, t, x); break;              \
+    case 2: __acpp_sscp_atomic_fetch_xor_##tag(as, o, s, t, x); break;             \
+    case 3: __acpp_sscp_atomic_fetch_add_##tag(as, o, s, t, x); break;             \
+    case 4: __acpp_sscp_atomic_fetch_sub_##tag(as, o, s, t, x); break;             \
+    case 5: __acpp_sscp_atomic_fetch_min_##tag(as, o, s, t, x); break;             \
+    default: __acpp_sscp_atomic_fetch_max_##tag(as, o, s, t, x); break;            \
+  }
+#define TK_NARROW_SWITCH_ARITH(tag)                                                \
+  switch (op) {                                                                    \
+    case 3: __acpp_sscp_atomic_fetch_add_##tag(as, o, s, t, x); break;             \
+    case 4: __acpp_sscp_atomic_fetch_sub_##tag(as, o, s, t, x); break;             \
+    case 5: __acpp_sscp_atomic_fetch_min_##tag(as, o, s, t, x); break;             \
+    default: __acpp_sscp_atomic_fetch_max_##tag(as, o, s, t, x); break;            \
+  }
+template <class T>
+__global__ void narrow_fold_kernel(int op, T *data, uint64_t n, uint32_t slots) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < slots || i >= n) return;
+  const T x = data[i];
+  T *t = data + (i % slots);
+  const auto as = acpp::address_space::global_space;
+  const auto o = acpp::memory_order::relaxed;
+  const auto s = acpp::memory_scope::device;
+  if constexpr (std::is_same_v<T, int8_t>) {
+    TK_NARROW_SWITCH(i8)
+  } else if constexpr (std::is_same_v<T, int16_t>) {
+    TK_NARROW_SWITCH(i16)
+  } else if constexpr (std::is_same_v<T, uint8_t>) {
+    TK_NARROW_SWITCH_ARITH(u8)
+  } else {
+    TK_NARROW_SWITCH_ARITH(u16)
+  }
+}
+#undef TK_NARROW_SWITCH
+#undef TK_NARROW_SWITCH_ARITH
+
+// One work-item: the tests/pstl/std_atomic.cpp sequence on data[k] of a
+// 4-element array whose other elements must stay as they are: load + store
+// (x + 1), exchange(1) + store(old + 1), and compare-exchange with a wrong
+// and a right expected value.  prev[0..5] receive the observed values.
+template <class T>
+__global__ void narrow_single_kernel(T *data, int k, T operand, T *prev) {
+  const auto as = acpp::address_space::global_space;
+  const auto o = acpp::memory_order::seq_cst;
+  const auto s = acpp::memory_scope::device;
+  T *p = data + k;
+  if constexpr (std::is_same_v<T, int8_t>) {
+    const T a = __acpp_sscp_atomic_load_i8(as, o, s, p);
+    __acpp_sscp_atomic_store_i8(as, o, s, p, static_cast<T>(a + 1));
+    const T b = __acpp_sscp_atomic_exchange_i8(as, o, s, p, 1);
+    __acpp_sscp_atomic_store_i8(as, o, s, p, static_cast<T>(b + 1));
+    T e0 = 0, e1 = static_cast<T>(b + 1);
+    const bool ok0 = __acpp_sscp_cmp_exch_strong_i8(as, o, o, s, p, &e0, operand);
+    const bool ok1 = __acpp_sscp_cmp_exch_weak_i8(as, o, o, s, p, &e1, operand);
+    prev[0] = a; prev[1] = b; prev[2] = e0; prev[3] = e1;
+    prev[4] = static_cast<T>(ok0); prev[5] = static_cast<T>(ok1);
+  } else {
+    const T a = __acpp_sscp_atomic_load_i16(as, o, s, p);
+    __acpp_sscp_atomic_store_i16(as, o, s, p, static_cast<T>(a + 1));
+    const T b = __acpp_sscp_atomic_exchange_i16(as, o, s, p, 1);
+    __acpp_sscp_atomic_store_i16(as, o, s, p, static_cast<T>(b + 1));
+    T e0 = 0, e1 = static_cast<T>(b + 1);
+    const bool ok0 = __acpp_sscp_cmp_exch_strong_i16(as, o, o, s, p, &e0, operand);
+    const bool ok1 = __acpp_sscp_cmp_exch_weak_i16(as, o, o, s, p, &e1, operand);
+    prev[0] = a; prev[1] = b; prev[2] = e0; prev[3] = e1;
+    prev[4] = static_cast<T>(ok0); prev[5] = static_cast<T>(ok1);
+  }
+}
+
 // tests/pstl/std_atomic.cpp:91-136: one work-item applies one operation and
 // records the previous value; also load/store/exchange/compare_exchange.
 template <class T>
@@ -422,6 +498,34 @@ int tk_atomic_single(int tag, int op, void *data, const void *operand, void *pre
     default: return 1;
   }
   return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+// 8/16-bit builtins.  tag: 7 i8, 8 i16, 9 u8, 10 u16.
+int tk_atomic_fold_narrow(int tag, int op, void *data, uint64_t n, uint32_t slots, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 block(256), grid(static_cast<uint32_t>((n + 255) / 256));
+  switch (tag) {
+    case 7: hipLaunchKernelGGL(narrow_fold_kernel<int8_t>, grid, block, 0, s, op, static_cast<int8_t *>(data), n, slots); break;
+    case 8: hipLaunchKernelGGL(narrow_fold_kernel<int16_t>, grid, block, 0, s, op, static_cast<int16_t *>(data), n, slots); break;
+    case 9: hipLaunchKernelGGL(narrow_fold_kernel<uint8_t>, grid, block, 0, s, op, static_cast<uint8_t *>(data), n, slots); break;
+    case 10: hipLaunchKernelGGL(narrow_fold_kernel<uint16_t>, grid, block, 0, s, op, static_cast<uint16_t *>(data), n, slots); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int tk_atomic_single_narrow(int tag, void *data, int k, int operand, void *prev, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (k < 0 || k > 3) return -1;
+  if (tag == 7)
+    hipLaunchKernelGGL(narrow_single_kernel<int8_t>, dim3(1), dim3(1), 0, s, static_cast<int8_t *>(data), k,
+                       static_cast<int8_t>(operand), static_cast<int8_t *>(prev));
+  else if (tag == 8)
+    hipLaunchKernelGGL(narrow_single_kernel<int16_t>, dim3(1), dim3(1), 0, s, static_cast<int16_t *>(data), k,
+                       static_cast<int16_t>(operand), static_cast<int16_t *>(prev));
+  else
+    return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 }  // extern "C"

@@ -36,6 +36,9 @@ def test_facade_signatures_compile(tmp_path):
   alg::reduce(q, g, fx, fx + 10, fx, 0.f, acpp::minimum<float>{});
   alg::transform_reduce(q, g, fx, fx + 10, fx, fx, 0.f);
   alg::transform_reduce(q, g, fx, fx + 10, fx, fx, 0.f, std::plus<float>{}, std::multiplies<float>{});
+  alg::transform_reduce(q, g, x, x + 10, x, y, 0, acpp::maximum<int>{}, acpp::absolute_difference<>{});
+  alg::transform_reduce(q, g, fx, fx + 10, fx, fx, 0.f, std::plus<float>{}, acpp::squared_difference<float>{});
+  alg::transform_reduce(q, g, lx, lx + 10, lx, lx, 0L, acpp::minimum<long>{}, std::minus<long>{}, deps);
   alg::inclusive_scan(q, g, x, x + 10, y);
   alg::inclusive_scan(q, g, x, x + 10, y, std::bit_xor<int>{});
   alg::inclusive_scan(q, g, x, x + 10, y, acpp::plus<int>{}, 7, deps);
@@ -56,8 +59,12 @@ def test_facade_signatures_compile(tmp_path):
     ("alg::reduce(q, g, x, x + 10, y, 0, [](int a, int b) { return a + b; });", "operator must be"),
     ("alg::reduce(q, g, fx, fx + 10, fx, 0.f, std::bit_and<float>{});", "floating-point types take"),
     ("alg::reduce(q, g, fx, fx + 10, fx, 0.f, acpp::bit_or<float>{});", "floating-point types take"),
-    ("alg::transform_reduce(q, g, x, x + 10, x, y, 0, acpp::maximum<int>{}, std::multiplies<int>{});",
-     "reduce = plus with"),
+    ("alg::transform_reduce(q, g, x, x + 10, x, y, 0, acpp::bit_and<int>{}, std::multiplies<int>{});",
+     "reduces with plus, multiplies"),
+    ("alg::transform_reduce(q, g, x, x + 10, x, y, 0, std::plus<int>{}, [](int a, int b) { return a - b; });",
+     "pair transform must be"),
+    ("alg::transform_reduce(q, g, fx, fx + 10, fx, fx, 0.f, std::plus<float>{}, std::bit_xor<float>{});",
+     "bitwise pair transforms need"),
     ("alg::reduce(q, g, x, x + 10, lx, 0L);", "input value type must equal T"),
     ("alg::transform_reduce(q, g, x, x + 10, y, 0, std::plus<int>{}, [](int v) { return v; });",
      "element transform must be"),
@@ -108,6 +115,64 @@ long f(std::vector<long, A<long>>& v, std::vector<long, A<long>>& w) {
 @pytest.mark.gpu
 def test_stdpar_on_gpu():
     r = subprocess.run([STDPAR_BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout
+
+
+RED_BIN = os.path.join(ROOT, "tests", "lib", "test_reduction_copy_if")
+
+RED_PRELUDE = """
+#include "acpp/reduction.hpp"
+#include "acpp/algorithms/algorithm.hpp"
+namespace alg = acpp::algorithms;
+void f(acpp::queue& q, alg::util::allocation_group& g, int* x, int* y, float* fx, long* lx,
+       std::size_t* cnt) {
+  const acpp::property_list init{acpp::property::reduction::initialize_to_identity{}};
+"""
+
+
+def test_reduction_and_copy_if_signatures_compile(tmp_path):
+    # reduction.hpp:165-236 factories, handler.hpp:799-975 fused form,
+    # algorithm.hpp:178-187 copy_if
+    r = _syntax(RED_PRELUDE + """
+  acpp::parallel_for(q, g, 10, acpp::combine_each(x, acpp::reduction(y, acpp::plus<int>{})));
+  acpp::parallel_for(q, g, 10, acpp::combine_each(x, acpp::reduction(y, std::plus<int>{}, init)));
+  acpp::parallel_for(q, g, 10, acpp::combine_each(x, acpp::reduction(y, 0, acpp::plus<int>{}, init)), {});
+  acpp::parallel_for(q, g, 10, acpp::combine_each(fx, acpp::reduction(fx, 1.f, acpp::multiplies<float>{})));
+  acpp::parallel_for(q, g, 10, acpp::combine_each(x, acpp::reduction(y, acpp::plus<int>{})),
+                     acpp::combine_each(x, acpp::reduction(y + 1, 5, acpp::maximum<int>{}, init)));
+  alg::copy_if(q, g, x, x + 10, y, acpp::is_greater_than(3));
+  alg::copy_if(q, g, x, x + 10, y, acpp::has_bits(1, 0), cnt);
+  alg::copy_if(q, g, fx, fx + 10, fx, acpp::is_nonzero<>{}, cnt, {});
+  alg::copy_if(q, g, lx, lx + 10, lx, acpp::is_less_equal(7L), nullptr);
+}
+""", tmp_path)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("call,message", [
+    ("alg::copy_if(q, g, x, x + 10, y, [](int v) { return v > 0; });", "the predicate must be"),
+    ("alg::copy_if(q, g, x, x + 10, y, acpp::is_greater_than(3L));", "the predicate must be"),
+    ("alg::copy_if(q, g, fx, fx + 10, fx, acpp::has_bits(1.f, 0.f));", "has_bits needs an integer"),
+    ("acpp::parallel_for(q, g, 10, acpp::combine_each(x, acpp::reduction(y, [](int a, int b) { return a + b; })));",
+     "operator must be"),
+    ("acpp::parallel_for(q, g, 10, acpp::combine_each(fx, acpp::reduction(fx, acpp::bit_or<float>{})));",
+     "floating-point types take"),
+])
+def test_reduction_and_copy_if_reject_at_compile_time(tmp_path, call, message):
+    r = _syntax(RED_PRELUDE + call + "\n}\n", tmp_path)
+    assert r.returncode != 0
+    assert message in r.stderr, r.stderr[-2000:]
+
+
+def test_reduction_copy_if_binary_built():
+    assert os.path.exists(RED_BIN), "run __graft_entry__.build() first"
+
+
+@pytest.mark.gpu
+def test_reduction_and_copy_if_on_gpu():
+    r = subprocess.run([RED_BIN], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL PASSED" in r.stdout

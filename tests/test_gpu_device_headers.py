@@ -45,6 +45,9 @@ def tk():
                                        ctypes.c_uint32, vp]
     lib.tk_atomic_fold.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_uint64, ctypes.c_int, vp]
     lib.tk_atomic_single.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+    lib.tk_atomic_fold_narrow.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_uint64,
+                                          ctypes.c_uint32, vp]
+    lib.tk_atomic_single_narrow.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp]
     return lib
 
 
@@ -311,3 +314,56 @@ def test_std_atomic_cases(gpu, tk, dtype):
         if op == 9:
             p = host(prev)
             assert list(p[1:5]) == [42, 42, 0, 1]
+
+
+NARROW_TAGS = {np.int8: 7, np.int16: 8, np.uint8: 9, np.uint16: 10}
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.int16, np.uint8, np.uint16])
+@pytest.mark.parametrize("slots", [1, 3, 4])
+def test_narrow_atomic_builtins_fold(gpu, tk, dtype, slots):
+    # __acpp_sscp_atomic_fetch_<op>_{i8,i16,u8,u16} (sscp/builtins/atomic.hpp:
+    # 110-311): 4096 work-items fold their element into data[i % slots];
+    # with 3 or 4 slots, neighbouring elements of one 32-bit word change
+    # concurrently and must not disturb each other.
+    torch, _ = gpu
+    n = 4096
+    rng = np.random.default_rng(slots)
+    info = np.iinfo(dtype)
+    vals = rng.integers(info.min, info.max, n, endpoint=True, dtype=dtype)
+    names = ["add", "sub", "min", "max"]
+    if dtype in (np.int8, np.int16):
+        names += ["and", "or", "xor"]
+    for name in names:
+        d = dev(torch, vals)
+        assert tk.tk_atomic_fold_narrow(NARROW_TAGS[dtype], RMW[name], d.data_ptr(), n, slots,
+                                        _stream(torch)) == 0
+        torch.cuda.synchronize()
+        got = host(d)
+        code = {"and": O.RMW_AND, "or": O.RMW_OR, "xor": O.RMW_XOR, "add": O.RMW_ADD,
+                "sub": O.RMW_SUB, "min": O.RMW_MIN, "max": O.RMW_MAX}[name]
+        for k in range(slots):
+            exp = O.atomic_fold(code, vals[slots + k::slots], vals[k])
+            assert got[k] == exp, (dtype, name, k)
+        assert np.array_equal(got[slots:], vals[slots:])  # operands untouched
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.int16])
+def test_narrow_atomic_builtins_single(gpu, tk, dtype):
+    # load / store / exchange / compare-exchange on one element of a word
+    # whose other elements must keep their values
+    torch, _ = gpu
+    for k in range(4):
+        init = np.array([10, 20, 30, 40], dtype=dtype)
+        d = dev(torch, init)
+        prev = torch.zeros(6, dtype=d.dtype, device="cuda")
+        assert tk.tk_atomic_single_narrow(NARROW_TAGS[dtype], d.data_ptr(), k, -7, prev.data_ptr(),
+                                          _stream(torch)) == 0
+        torch.cuda.synchronize()
+        x0 = int(init[k])
+        # load -> x0; store x0 + 1; exchange(1) -> x0 + 1; store x0 + 2;
+        # cas(expected 0) fails and sees x0 + 2; cas(expected x0 + 2) stores -7
+        assert list(host(prev)) == [x0, x0 + 1, x0 + 2, x0 + 2, 0, 1], (dtype, k)
+        want = init.copy()
+        want[k] = -7
+        assert np.array_equal(host(d), want), (dtype, k)

@@ -6,18 +6,18 @@ check them against the oracle: bit-exact for integer work, within the stated
 bound against an FP64 oracle for floating point.  Inputs are generated on the
 GPU (seeded torch generators) and copied back for the oracle.
 """
-import math
-
 import numpy as np
 import pytest
 
 import opensycl_amd as acpp
 import oracle as O
-from gpu_util import host, lognormal_offsets
+from gpu_util import (compensated_bound, exact_ints, fold_bound, host, lognormal_offsets,
+                      reduce_depth, scan_depth)
 
 pytestmark = pytest.mark.gpu
 
 N = 1 << 28
+EPS32 = 2.0 ** -24
 
 
 def _gen(torch, seed):
@@ -26,9 +26,12 @@ def _gen(torch, seed):
     return g
 
 
-def _fp_bound(abs_sum, n, eps):
-    # a fold of depth at most d roundings: |err| <= d * eps * sum|x|
-    return (64 + 2 * int(math.log2(max(n, 2)))) * eps * abs_sum
+def _fp_bound(abs_sum, n):
+    return fold_bound(abs_sum, reduce_depth(n, acpp.compute_units()), EPS32)
+
+
+def _comp_bound(exact, abs_sum, n):
+    return compensated_bound(exact, abs_sum, n, acpp.compute_units())
 
 
 def _stress(torch, seed):
@@ -52,8 +55,8 @@ def test_dot_f32_2e28(gpu, q, scratch):
     ah, bh = host(a), host(b)
     exact = O.dot_fp64(ah, bh)
     abs_sum = float(np.abs(ah.astype(np.float64) * bh.astype(np.float64)).sum())
-    assert abs(float(out.item()) - exact) <= _fp_bound(abs_sum, N, 2.0 ** -24)
-    assert abs(float(comp.item()) - exact) <= 2.0 ** -24 * abs(exact) + 2.0 ** -40 * abs_sum
+    assert abs(float(out.item()) - exact) <= _fp_bound(abs_sum, N)
+    assert abs(float(comp.item()) - exact) <= _comp_bound(exact, abs_sum, N)
 
 
 def test_precision_sums_2e28(gpu, q, scratch):
@@ -73,16 +76,83 @@ def test_precision_sums_2e28(gpu, q, scratch):
         q.wait()
         err = abs(float(out.item()) - exact)
         if compensated:
-            assert err <= 2.0 ** -24 * abs(exact) + 2.0 ** -40 * abs_sum
+            assert err <= _comp_bound(exact, abs_sum, N)
         else:
-            assert err <= _fp_bound(abs_sum, N, 2.0 ** -24)
+            assert err <= _fp_bound(abs_sum, N)
     xb = x.to(torch.bfloat16)
-    exact_b = O.sum_fp64((host(xb.view(torch.int16)).view(np.uint16)))
-    out = torch.zeros(1, dtype=torch.float32, device="cuda")
-    acpp.reduce_compensated(q, scratch, xb, out, 0.0)
-    q.wait()
-    # bf16 rounding changes each |x| by at most 2^-8 relative
-    assert abs(float(out.item()) - exact_b) <= 2.0 ** -24 * abs(exact_b) + 2.0 ** -40 * 1.01 * abs_sum
+    bits = host(xb.view(torch.int16)).view(np.uint16)
+    exact_b = O.sum_fp64(bits)
+    abs_b = O.sum_fp64(bits & np.uint16(0x7FFF))
+    for compensated in (False, True):
+        out = torch.zeros(1, dtype=torch.float32, device="cuda")
+        if compensated:
+            acpp.reduce_compensated(q, scratch, xb, out, 0.0)
+        else:
+            acpp.reduce(q, scratch, xb, out, 0.0)
+        q.wait()
+        err = abs(float(out.item()) - exact_b)
+        if compensated:
+            assert err <= _comp_bound(exact_b, abs_b, N)
+        else:
+            assert err <= _fp_bound(abs_b, N)
+
+
+def test_precision_dots_2e28(gpu, q, scratch):
+    # fp32 and bf16 dot products of 2^28 stressed values with 2^28 N(0,1):
+    # plain within the fold bound, compensated (Dot2) within one fp32
+    # rounding of the FP64 dot plus the second-order term
+    torch, _ = gpu
+    a = _stress(torch, 7)
+    b = torch.randn(N, generator=_gen(torch, 8), device="cuda", dtype=torch.float32)
+    ah, bh = host(a), host(b)
+    cases = [(a, b, O.dot_fp64(ah, bh), O.dot_fp64(np.abs(ah), np.abs(bh)))]
+    ab, bb = a.to(torch.bfloat16), b.to(torch.bfloat16)
+    abits = host(ab.view(torch.int16)).view(np.uint16)
+    bbits = host(bb.view(torch.int16)).view(np.uint16)
+    cases.append((ab, bb, O.dot_fp64(abits, bbits),
+                  O.dot_fp64(abits & np.uint16(0x7FFF), bbits & np.uint16(0x7FFF))))
+    for x1, x2, exact, abs_sum in cases:
+        for compensated in (False, True):
+            out = torch.zeros(1, dtype=torch.float32, device="cuda")
+            acpp.transform_reduce(q, scratch, x1, x2, out, 0.0, compensated=compensated)
+            q.wait()
+            err = abs(float(out.item()) - exact)
+            bound = _comp_bound(exact, abs_sum, N) if compensated else _fp_bound(abs_sum, N)
+            assert err <= bound, (x1.dtype, compensated, err, bound)
+
+
+def test_exact_sums_and_dots_2e28(gpu, q, scratch):
+    # Exactly summable inputs: non-zero integers in +-[1, 8] (exact in fp32
+    # and bf16).  Every sum and dot product, plain or compensated, must hit
+    # the integer result exactly, so one lost element (or one lost 16-byte
+    # vector, unless its elements cancel) fails the test.
+    torch, _ = gpu
+    xi = exact_ints(torch, N, 9)
+    yi = exact_ints(torch, N, 10)
+    xh = host(xi).astype(np.int64)
+    yh = host(yi).astype(np.int64)
+    want_sum = int(O.reduce(xh, np.int64(0)))
+    want_dot = int(O.dot(xh, yh, np.int64(0)))
+    del xh, yh
+    for dt in (torch.float32, torch.bfloat16):
+        x, y = xi.to(dt), yi.to(dt)
+        for compensated in (False, True):
+            out = torch.zeros(1, dtype=torch.float32, device="cuda")
+            if compensated:
+                acpp.reduce_compensated(q, scratch, x, out, 0.0)
+            else:
+                acpp.reduce(q, scratch, x, out, 0.0)
+            dot = torch.zeros(1, dtype=torch.float32, device="cuda")
+            acpp.transform_reduce(q, scratch, x, y, dot, 0.0, compensated=compensated)
+            q.wait()
+            assert float(out.item()) == want_sum, (dt, compensated)
+            assert float(dot.item()) == want_dot, (dt, compensated)
+        # a view that starts one element in, so the scalar head is used too
+        out = torch.zeros(1, dtype=torch.float32, device="cuda")
+        acpp.reduce(q, scratch, x[1:], out, 0.0)
+        q.wait()
+        assert float(out.item()) == want_sum - int(xi[0].item()), dt
+        del x, y
 
 
 @pytest.mark.parametrize("num_bins", [4096, 65536])
@@ -138,5 +208,14 @@ def test_segmented_reduce_lognormal_workload(gpu, q, scratch):
     lens = np.diff(off.astype(np.int64))
     assert lens.min() >= 1  # reduceat below needs non-empty segments
     abs_sums = np.add.reduceat(np.abs(xh.astype(np.float64)), off[:-1].astype(np.int64))
-    bound = (64 + 2 * np.log2(lens + 2)) * 2.0 ** -24 * abs_sums * 1.0001
+    # segment partials of whole tiles chained by the fix-up: one level per
+    # tile the segment spans, on top of the in-tile depth
+    bound = fold_bound(abs_sums, scan_depth(lens, 1024), EPS32)
     assert np.all(np.abs(got - exact) <= bound)
+    # exactly summable values: every segment sum must be exact
+    xi = exact_ints(torch, n, 11)
+    xe = xi.to(torch.float32)
+    acpp.segmented_reduce(q, scratch, xe, od, out, 0.0)
+    q.wait()
+    want = O.segmented_reduce(host(xi).astype(np.int64), off, np.int64(0))
+    assert np.array_equal(host(out).astype(np.int64), want)

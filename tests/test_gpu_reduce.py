@@ -10,7 +10,8 @@ import pytest
 
 import opensycl_amd as acpp
 import oracle as O
-from gpu_util import ALL_OPS, FP_OPS, INT_TYPES, dev, host, op_input, rand
+from gpu_util import (ALL_OPS, FP_OPS, INT_TYPES, compensated_bound, dev, fold_bound, host, op_input,
+                      rand, reduce_depth)
 
 pytestmark = pytest.mark.gpu
 
@@ -117,9 +118,9 @@ def test_reduction_semantics(gpu, q, scratch, dtype):
                 q.wait()
                 got = host(out)[0]
                 if np.issubdtype(dtype, np.floating) and op in (0, 1):
-                    # FP64 reference; the kernel's fold has depth <= 64 + 2 log2 n
+                    # FP64 reference within the kernel's fold bound
                     eps = 2.0 ** -24 if dtype == np.float32 else 2.0 ** -53
-                    depth = 64 + 2 * int(math.log2(max(n, 2)))
+                    depth = reduce_depth(n, acpp.compute_units()) + 1
                     x64 = x.astype(np.float64)
                     if op == 0:
                         exact = math.fsum(x64) + (0.0 if to_id else float(start))
@@ -135,9 +136,11 @@ def test_reduction_semantics(gpu, q, scratch, dtype):
 
 
 def fp_sum_bound(x64_abs_sum, n, eps):
-    # a fold with at most depth d roundings: |err| <= d * eps * sum|x|
-    depth = 64 + int(math.log2(max(n, 2))) * 2
-    return depth * eps * x64_abs_sum
+    return fold_bound(x64_abs_sum, reduce_depth(n, acpp.compute_units()), eps)
+
+
+def comp_bound(exact, abs_sum, n):
+    return compensated_bound(exact, abs_sum, n, acpp.compute_units())
 
 
 @pytest.mark.parametrize("n", [1, 1000, (1 << 20) + 5, 1 << 24])
@@ -181,7 +184,7 @@ def test_reduce_f64_plus(gpu, q, scratch):
     acpp.reduce(q, scratch, dev(torch, x), out, 0.0)
     q.wait()
     exact = math.fsum(x)
-    assert abs(host(out)[0] - exact) <= 200 * 2.0 ** -53 * np.abs(x).sum()
+    assert abs(host(out)[0] - exact) <= fp_sum_bound(np.abs(x).sum(), x.size, 2.0 ** -53)
 
 
 def _stress_f32(n, seed):
@@ -207,14 +210,14 @@ def test_compensated_sum_f32_and_bf16(gpu, q, scratch, n):
     q.wait()
     err = abs(float(host(out)[0]) - exact)
     # one final rounding to fp32 plus a second-order term
-    assert err <= 2.0 ** -24 * abs(exact) + 2.0 ** -40 * abs_sum + 1e-30
+    assert err <= comp_bound(exact, abs_sum, n) + 1e-30
     b = _bf16_bits(x)
     exact_b = O.sum_fp64(b)
     bt = dev(torch, b).view(torch.bfloat16)
     out = _out(torch, np.float32)
     acpp.reduce_compensated(q, scratch, bt, out, 0.0)
     q.wait()
-    assert abs(float(host(out)[0]) - exact_b) <= 2.0 ** -24 * abs(exact_b) + 2.0 ** -40 * abs_sum + 1e-30
+    assert abs(float(host(out)[0]) - exact_b) <= comp_bound(exact_b, abs_sum, n) + 1e-30
     # plain fp32 accumulation of bf16 input
     out = _out(torch, np.float32)
     acpp.reduce(q, scratch, bt, out, 0.0)
@@ -267,7 +270,7 @@ def test_dot_fp(gpu, q, scratch, n):
     out = _out(torch, np.float32)
     acpp.transform_reduce(q, scratch, ad, bd, out, 0.0, compensated=True)
     q.wait()
-    assert abs(float(host(out)[0]) - exact) <= 2.0 ** -24 * abs(exact) + 2.0 ** -40 * abs_sum
+    assert abs(float(host(out)[0]) - exact) <= comp_bound(exact, abs_sum, n)
     ab, bb = _bf16_bits(a), _bf16_bits(b)
     exact_b = O.dot_fp64(ab, bb)
     abt = dev(torch, ab).view(torch.bfloat16)
@@ -275,7 +278,7 @@ def test_dot_fp(gpu, q, scratch, n):
     out = _out(torch, np.float32)
     acpp.transform_reduce(q, scratch, abt, bbt, out, 0.0, compensated=True)
     q.wait()
-    assert abs(float(host(out)[0]) - exact_b) <= 2.0 ** -24 * abs(exact_b) + 2.0 ** -40 * abs_sum
+    assert abs(float(host(out)[0]) - exact_b) <= comp_bound(exact_b, abs_sum, n)
     out = _out(torch, np.float32)
     acpp.transform_reduce(q, scratch, abt, bbt, out, 0.0)
     q.wait()

@@ -4,14 +4,14 @@ Integer scans must be bit-exact for every operator, size, alignment and
 launch configuration; floating-point sums are compared with an FP64 prefix
 sum within the bound stated in the test.
 """
-import math
 
 import numpy as np
 import pytest
 
 import opensycl_amd as acpp
 import oracle as O
-from gpu_util import ALL_OPS, INT_TYPES, dev, host, lognormal_offsets, op_input, rand
+from gpu_util import (ALL_OPS, INT_TYPES, dev, fold_bound, host, lognormal_offsets, op_input, rand,
+                      scan_depth)
 
 pytestmark = pytest.mark.gpu
 
@@ -113,8 +113,8 @@ def test_scan_fp_sum_vs_fp64(gpu, q, scratch, dtype, eps):
         got = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive").astype(np.float64)
         x64 = x.astype(np.float64)
         exact = np.cumsum(x64)
-        depth = 64 + 2 * int(math.log2(max(n, 2)))
-        bound = depth * eps * np.cumsum(np.abs(x64)) + 1e-300
+        # tiles hold at least 256 x 8 x 16 bytes (4096 doubles)
+        bound = fold_bound(np.cumsum(np.abs(x64)), scan_depth(np.arange(n), 4096), eps) + 1e-300
         # np.cumsum in FP64 differs from the exact prefix by far less than eps(f32)
         assert np.all(np.abs(got - exact) <= bound)
 

@@ -7,7 +7,10 @@ import pytest
 
 import opensycl_amd as acpp
 import oracle as O
-from gpu_util import dev, host, lognormal_offsets, rand
+from gpu_util import dev, exact_ints, fold_bound, host, lognormal_offsets, rand, scan_depth
+
+# segmented tiles hold at least 128 work-items x 2 rows x 16 bytes
+MIN_TILE = 512
 
 pytestmark = pytest.mark.gpu
 
@@ -50,11 +53,33 @@ def test_segmented_scan(gpu, q, scratch, dtype):
                     b, e = int(off[s]), int(off[s + 1])
                     if e > b:
                         ex = np.cumsum(x64[b:e])
-                        bound = (64 + 2 * math.log2(e - b + 2)) * eps * np.cumsum(np.abs(x64[b:e]))
+                        bound = fold_bound(np.cumsum(np.abs(x64[b:e])),
+                                           scan_depth(np.arange(e - b), MIN_TILE), eps)
                         ok &= bool(np.all(np.abs(got[b:e].astype(np.float64) - ex) <= bound + 1e-300))
                 assert ok, (name, op)
             else:
                 assert np.array_equal(got, O.segmented_inclusive_scan(x, off, op)), (name, op)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_segmented_exact_fp(gpu, q, scratch, dtype):
+    # integer-valued floats: per-segment sums and prefix sums are exact, so
+    # the floating-point kernels must agree bit for bit with the oracle's
+    # integer scan and fold; a lost or duplicated element fails here
+    torch, _ = gpu
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    for name, off in layouts():
+        n = int(off[-1])
+        xi = exact_ints(torch, n, n % 1000)
+        xh = host(xi).astype(np.int64)
+        xd, od = xi.to(tdt), dev(torch, off)
+        yd = torch.zeros_like(xd)
+        acpp.segmented_inclusive_scan(q, scratch, xd, yd, od)
+        out = torch.zeros(len(off) - 1, dtype=tdt, device="cuda")
+        acpp.segmented_reduce(q, scratch, xd, od, out, 0.0)
+        q.wait()
+        assert np.array_equal(host(yd).astype(np.int64), O.segmented_inclusive_scan(xh, off)), name
+        assert np.array_equal(host(out).astype(np.int64), O.segmented_reduce(xh, off, np.int64(0))), name
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64, np.float32, np.float64])
@@ -77,7 +102,7 @@ def test_segmented_reduce(gpu, q, scratch, dtype):
                 absx = np.abs(x.astype(np.float64))
                 abs_sums = np.array([absx[int(off[s]):int(off[s + 1])].sum() for s in range(S)])
                 eps = np.finfo(dtype).eps
-                bound = (64 + 2 * np.log2(lens + 2)) * eps * (abs_sums + 7) + 1e-300
+                bound = fold_bound(abs_sums + 7, scan_depth(lens, MIN_TILE) + 1, eps) + 1e-300
                 assert np.all(np.abs(got.astype(np.float64) - (exact + 7)) <= bound), name
             else:
                 assert np.array_equal(got, O.segmented_reduce(x, off, init, op)), (name, op)
@@ -123,7 +148,7 @@ def test_segmented_plan_replays(gpu, q, scratch):
     exact = O.segmented_reduce_f32_fp64(f, off)
     lens = np.diff(off.astype(np.int64))
     abs_sums = O.segmented_reduce_f32_fp64(np.abs(f), off)
-    bound = (64 + 2 * np.log2(lens + 2)) * np.finfo(np.float32).eps * abs_sums + 1e-30
+    bound = fold_bound(abs_sums, scan_depth(lens, MIN_TILE), np.finfo(np.float32).eps) + 1e-30
     for rep in range(4):
         yd.zero_()
         red.zero_()
@@ -200,3 +225,32 @@ def test_segmented_views_one_element_off(gpu, q, scratch, dtype):
     q.wait()
     assert np.array_equal(host(yb)[1:], O.segmented_inclusive_scan(x[1:], off))
     assert np.array_equal(host(out), O.segmented_reduce(x[1:], off, dtype(0), acpp.PLUS))
+
+
+def test_plan_keeps_its_group_and_buffers_alive(gpu, q):
+    # A plan built from a temporary group and temporary tensors: after the
+    # builder's references are gone and the allocator and cache have been
+    # churned, a replay still scans the right data into the right output.
+    import gc
+    torch, cache = gpu
+    off = lognormal_offsets(500, seed=9, max_len=1 << 14)
+    n = int(off[-1])
+    x = rand(np.int64, n, seed=9, lo=-1000, hi=1000)
+
+    def make():
+        g = acpp.AllocationGroup(cache, 0)
+        xd, od = dev(torch, x), dev(torch, off)
+        yd = torch.zeros_like(xd)
+        return acpp.SegmentedPlan.inclusive_scan(g, xd, yd, od), yd
+
+    plan, yd = make()
+    gc.collect()
+    junk = [torch.full((n,), 7, dtype=torch.int64, device="cuda") for _ in range(3)]
+    with acpp.AllocationGroup(cache, 0) as other:
+        acpp.inclusive_scan(q, other, junk[0], junk[1])
+        plan.launch(q)
+        acpp.inclusive_scan(q, other, junk[1], junk[2])
+        q.wait()
+    assert np.array_equal(host(yd), O.segmented_inclusive_scan(x, off))
+    assert int(host(junk[2])[-1]) == 7 * n * (n + 1) // 2  # n < 2^23: no wrap
+    plan.close()

@@ -225,3 +225,55 @@ def test_transform_reduce_launch_configs(gpu, q, scratch):
     exp = O.transform_reduce(x, np.int32(0), O.PLUS, O.ABS)
     for cfg in [(64, 4), (256, 4, 1), (1024, 4, 2), (512, 4, 0, 0)]:
         assert _reduce(torch, q, scratch, x, np.int32(0), acpp.PLUS, acpp.ABS, config=cfg) == exp, cfg
+
+
+# ---------------------------------------------------------------------------
+# Binary transform_reduce (numeric.hpp:170-191) with pair transforms.
+# ---------------------------------------------------------------------------
+BIN_INT_TYPES = [np.int32, np.int64, np.uint32, np.uint64]
+
+
+@pytest.mark.parametrize("dtype", BIN_INT_TYPES + FP_TYPES)
+def test_binary_transform_reduce(gpu, q, scratch, dtype):
+    # integers: full-range values, bit-exact for every pair transform and
+    # reduction; floating point: integers in [-2, 2], whose pair transforms
+    # are at most 16 in magnitude, so every sum of up to 100003 of them is
+    # below 2^24 and exact in any order
+    torch, _ = gpu
+    is_int = np.issubdtype(dtype, np.integer)
+    codes = range(10) if is_int else range(7)
+    ops = (acpp.PLUS, acpp.MULTIPLY, acpp.MIN, acpp.MAX) if is_int else (acpp.PLUS, acpp.MIN, acpp.MAX)
+    for n in ((1, 1000, (1 << 20) + 3) if is_int else (1, 1000, 100003)):
+        rng = np.random.default_rng(n)
+        if is_int:
+            a, b = rand(dtype, n + 1, seed=n), rand(dtype, n + 1, seed=n + 1)
+        else:
+            a = rng.integers(-2, 3, n + 1).astype(dtype)
+            b = rng.integers(-2, 3, n + 1).astype(dtype)
+        ad, bd = dev(torch, a), dev(torch, b)
+        for code in codes:
+            for op in ops:
+                for off in (0, 1):  # views one element in: the scalar head
+                    out = torch.full((1,), 77, dtype=ad.dtype, device="cuda")
+                    acpp.transform_reduce(q, scratch, ad[off:off + n], bd[off:off + n], out, dtype(5),
+                                          reduce_op=op, transform=code)
+                    q.wait()
+                    want = O.transform_reduce_binary(a[off:off + n], b[off:off + n], dtype(5), op, code)
+                    assert host(out)[0] == want, (dtype, n, code, op, off)
+    # empty input leaves out untouched
+    out = torch.full((1,), 77, dtype=ad.dtype, device="cuda")
+    acpp.transform_reduce(q, scratch, ad[:0], bd[:0], out, dtype(5), reduce_op=acpp.MAX, transform=acpp.B_ABS_DIFF)
+    q.wait()
+    assert host(out)[0] == 77
+
+
+def test_binary_transform_reduce_rejects(gpu, q, scratch):
+    torch, _ = gpu
+    x = torch.zeros(8, dtype=torch.float32, device="cuda")
+    o = torch.zeros(1, dtype=torch.float32, device="cuda")
+    with pytest.raises(acpp.AcppError):
+        acpp.transform_reduce(q, scratch, x, x, o, 0.0, transform=acpp.B_BIT_AND, reduce_op=acpp.PLUS)
+    xi = torch.zeros(8, dtype=torch.int32, device="cuda")
+    oi = torch.zeros(1, dtype=torch.int32, device="cuda")
+    with pytest.raises(acpp.AcppError):
+        acpp.transform_reduce(q, scratch, xi, xi, oi, 0, transform=acpp.B_MINUS, reduce_op=acpp.BIT_XOR)

@@ -139,3 +139,101 @@ def test_planted_fault_is_caught():
     assert not np.array_equal(got, _np_transform(x, O.ABS))
     xf = _inputs(np.float32)
     assert not np.array_equal(O.transform(xf, O.ABS), _np_transform(xf, O.ABS))
+
+
+# ---------------------------------------------------------------------------
+# Binary transform_reduce (numeric.hpp:170-191): pinned against numpy's
+# wrapping ufuncs and closed forms; a planted |x-y| -> x-y must fail.
+# ---------------------------------------------------------------------------
+BIN_TYPES = [np.int32, np.int64, np.uint32, np.uint64, np.float32, np.float64]
+
+
+def np_pair(code, a, b):
+    with np.errstate(over="ignore"):
+        if code == O.B_MULTIPLIES:
+            return np.multiply(a, b, dtype=a.dtype)
+        if code == O.B_PLUS:
+            return np.add(a, b, dtype=a.dtype)
+        if code == O.B_MINUS:
+            return np.subtract(a, b, dtype=a.dtype)
+        if code == O.B_ABS_DIFF:
+            d = np.subtract(a, b, dtype=a.dtype)
+            return np.abs(d) if np.issubdtype(a.dtype, np.signedinteger) or \
+                np.issubdtype(a.dtype, np.floating) else d
+        if code == O.B_SQ_DIFF:
+            d = np.subtract(a, b, dtype=a.dtype)
+            return np.multiply(d, d, dtype=a.dtype)
+        if code == O.B_MIN:
+            return np.where(a < b, a, b)
+        if code == O.B_MAX:
+            return np.where(a > b, a, b)
+        if code == O.B_BIT_AND:
+            return np.bitwise_and(a, b)
+        if code == O.B_BIT_OR:
+            return np.bitwise_or(a, b)
+        return np.bitwise_xor(a, b)
+
+
+def np_fold(op, v, init):
+    x = np.concatenate([np.array([init], dtype=v.dtype), v])
+    with np.errstate(over="ignore"):
+        return {O.PLUS: lambda: np.add.reduce(x, dtype=x.dtype),
+                O.MULTIPLY: lambda: np.multiply.reduce(x, dtype=x.dtype),
+                O.MIN: lambda: np.minimum.reduce(x), O.MAX: lambda: np.maximum.reduce(x)}[op]()
+
+
+@pytest.mark.parametrize("dt", BIN_TYPES)
+def test_binary_transform_reduce_vs_numpy(dt):
+    rng = np.random.default_rng(11)
+    codes = range(10) if np.issubdtype(dt, np.integer) else range(7)
+    for n in (0, 1, 5, 1000):
+        if np.issubdtype(dt, np.integer):
+            info = np.iinfo(dt)
+            a = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+            b = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+        else:
+            # integer-valued, so every fold order gives the same bits
+            a = rng.integers(-50, 50, n).astype(dt)
+            b = rng.integers(-50, 50, n).astype(dt)
+        for code in codes:
+            for op in (O.PLUS, O.MIN, O.MAX) + ((O.MULTIPLY,) if np.issubdtype(dt, np.integer) else ()):
+                got = O.transform_reduce_binary(a, b, dt(3), op, code)
+                want = np_fold(op, np_pair(code, a, b), dt(3))
+                assert got == want, (dt, n, code, op)
+
+
+def check_binary_closed_forms():
+    i = np.arange(1, 1001, dtype=np.int64)
+    ok = O.transform_reduce_binary(i, 2 * i, np.int64(0), O.PLUS, O.B_ABS_DIFF) == 1000 * 1001 // 2
+    ok &= O.transform_reduce_binary(i, 2 * i, np.int64(0), O.PLUS, O.B_SQ_DIFF) == 1000 * 1001 * 2001 // 6
+    ok &= O.transform_reduce_binary(i, 3 * i, np.int64(7), O.MAX, O.B_ABS_DIFF) == 2000
+    f = i.astype(np.float64)
+    ok &= O.transform_reduce_binary(f, 2 * f, 0.0, O.PLUS, O.B_ABS_DIFF) == 500500.0
+    return bool(ok)
+
+
+def test_binary_transform_reduce_closed_forms():
+    assert check_binary_closed_forms()
+    O.set_fault(15)
+    try:
+        assert not check_binary_closed_forms()
+    finally:
+        O.set_fault(0)
+
+
+def test_binary_transform_reduce_dot_closed_form():
+    # plus with multiplies is the dot product (numeric.hpp:215-223):
+    # sum over i < n of i (i + 1) = (n-1) n (n+1) / 3, and oracle_dot agrees
+    for n in (1, 10, 1000):
+        a = np.arange(n, dtype=np.int64)
+        assert O.transform_reduce_binary(a, a + 1, np.int64(0), O.PLUS, O.B_MULTIPLIES) == \
+            (n - 1) * n * (n + 1) // 3
+        assert O.transform_reduce_binary(a, a + 1, np.int64(0), O.PLUS, O.B_MULTIPLIES) == O.dot(a, a + 1, 0)
+
+
+def test_binary_transform_reduce_rejects():
+    a = np.zeros(3, dtype=np.float32)
+    with pytest.raises(ValueError):
+        O.transform_reduce_binary(a, a, np.float32(0), O.PLUS, O.B_BIT_AND)
+    with pytest.raises(ValueError):
+        O.transform_reduce_binary(a.astype(np.int32), a.astype(np.int32), 0, O.PLUS, 10)
