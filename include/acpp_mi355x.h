@@ -355,6 +355,17 @@ acpp_status acpp_inclusive_scan_f64(acpp_allocation_group *scratch, acpp_stream_
 acpp_status acpp_exclusive_scan_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *d_first, int8_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *d_first, int16_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, int32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
+/* No counterpart in the reference.  acpp_exclusive_scan_i32 for callers
+   that cannot allocate (code under hipGraph capture, tight serving loops,
+   harnesses that forbid hipMalloc): no allocation group; the scan's sync
+   region (256-byte header + 2 MiB of tile status lines) is a static device
+   array of the library, one per GPU, set up on the first call on each
+   device.  At most 16384 tiles (2^29 elements with the default 1024 x 8-row
+   tiles); larger inputs are rejected with ACPP_ERROR_INVALID_ARGUMENT.
+   Calls on one GPU share that array: calls whose kernels overlap on one GPU
+   (different streams) race and give wrong results; calls ordered on a
+   stream, or otherwise serialised, are safe.  Calls no hipMalloc/hipFree. */
+acpp_status acpp_exclusive_scan_i32_noalloc(acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, int32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, int64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *d_first, uint8_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *d_first, uint16_t init, acpp_algorithm_op op, const acpp_launch_config *config);

@@ -62,6 +62,7 @@ def _signatures() -> dict[str, tuple[list, object]]:
     S["acpp_allocation_group_destroy"] = ([vp], c_int)
     S["acpp_allocation_group_watchdog_trips"] = ([vp, POINTER(c_uint64)], c_int)
     S["acpp_check_asynchronous_errors"] = ([c_int], c_int)
+    S["acpp_exclusive_scan_i32_noalloc"] = ([vp, vp, c_uint64, vp, c_int32, c_int32, cfg], c_int)
     S["acpp_stdpar_should_offload"] = ([c_int, c_int, c_int, c_int, c_uint64, vp, c_uint32, POINTER(c_int)], c_int)
     S["acpp_stdpar_set_offload_policy"] = ([c_int], c_int)
     S["acpp_stdpar_offload_threshold"] = ([c_int, c_int, c_int, c_int, POINTER(c_uint64)], c_int)
