@@ -1,45 +1,70 @@
 // Standard-parallelism entry points for the numeric algorithms: reduce,
 // transform_reduce, inclusive_scan, exclusive_scan, transform_inclusive_scan
-// and transform_exclusive_scan with a par_unseq policy, offloaded to the MI355X kernels through the acpp::algorithms
-// facade.
+// and transform_exclusive_scan with the par_unseq policy
+// (std::execution::parallel_unsequenced_policy), offloaded to the MI355X
+// kernels through the acpp::algorithms facade or run on the host.
 //
-// Same entry points and offload structure as the reference's stdpar layer
-// (include/hipSYCL/std/stdpar/pstl-impl/numeric.hpp:28-580): a thread-local
-// in-order queue and thread-local scratch caches
-// (include/hipSYCL/std/stdpar/detail/sycl_glue.hpp:54-90), device scratch
-// for the algorithm and host scratch for a reduction's result, and the
-// reduction returns once its result is on the host (the reference's
-// HIPSYCL_STDPAR_BLOCKING_OFFLOAD).
+// Same entry points, argument lists and offload structure as the reference's
+// stdpar layer (include/hipSYCL/std/stdpar/pstl-impl/numeric.hpp:28-580):
+//   - a thread-local in-order queue and thread-local scratch caches
+//     (include/hipSYCL/std/stdpar/detail/sycl_glue.hpp:54-90);
+//   - reductions block until their result is on the host
+//     (HIPSYCL_STDPAR_BLOCKING_OFFLOAD); scans return as soon as their work
+//     is enqueued (HIPSYCL_STDPAR_OFFLOAD, numeric.hpp:298-358).  The
+//     reference's compiler then inserts a synchronisation before the host
+//     touches the data; without that compiler pass the caller synchronises
+//     explicitly: acpp::stdpar::sync(), or any blocking call of this header
+//     (the queue is in order).
+//   - every call decides between the GPU and the host first
+//     (acpp_stdpar_should_offload: memory only the device can read forces
+//     the GPU, pageable host memory forces the host, and for pinned or
+//     managed memory the library's learned cost model picks the faster
+//     side; the reference interpolates sampled run times instead,
+//     offload.hpp:384-594, offload_heuristic_db.hpp:247-318).  The host side
+//     is the sequential C++ standard library algorithm, run after the
+//     thread's offloaded work has completed.
+// Element types, operators and transforms are those of the acpp::algorithms
+// facade (acpp/algorithms/numeric.hpp); others fail to compile.
 //
-// Two differences follow from having no compiler support here:
-//   - every call, scans included, completes before it returns.  The
-//     reference returns from a scan without waiting and relies on
-//     synchronisation points its compiler inserts before the host touches
-//     the data;
-//   - there is no host fallback and no offload heuristic: the data must be
-//     device-accessible (hipMalloc, hipMallocManaged or hipHostMalloc
-//     memory; the reference gets there by replacing the allocator), and
-//     every call runs on the GPU.
+// To make std::reduce(std::execution::par_unseq, ...) itself offload, put
+// include/acpp/stdpar/std before the system include directories: its
+// <numeric> adds std:: overloads that forward here, as the reference's
+// include/hipSYCL/std/stdpar/numeric does.
 #pragma once
 
 #include <cstddef>
+#include <execution>
 #include <functional>
 #include <iterator>
 #include <memory>
+#include <numeric>
+#include <type_traits>
 
 #include "acpp/algorithms/numeric.hpp"
 
 namespace acpp {
 namespace stdpar {
 
-struct par_unseq_policy {};
-inline constexpr par_unseq_policy par_unseq{};
+using par_unseq_policy = std::execution::parallel_unsequenced_policy;
+inline constexpr const par_unseq_policy &par_unseq = std::execution::par_unseq;
+
+// Process-wide override of the learned offload choice.
+enum class offload_policy : int {
+  automatic = ACPP_STDPAR_OFFLOAD_AUTO,
+  always = ACPP_STDPAR_OFFLOAD_ALWAYS,
+  never = ACPP_STDPAR_OFFLOAD_NEVER,
+};
+
+inline void set_offload_policy(offload_policy p) {
+  acpp::detail::check_status(acpp_stdpar_set_offload_policy(static_cast<int>(p)));
+}
 
 namespace detail {
 
 // Thread-local queue and scratch caches (sycl_glue.hpp:54-90).
 struct tls_runtime {
   acpp::queue q{0};
+  std::size_t offloaded = 0;  // calls of this thread that ran on the GPU
   algorithms::util::allocation_cache device_cache{algorithms::util::allocation_type::device};
   algorithms::util::allocation_cache host_cache{algorithms::util::allocation_type::host};
 
@@ -49,13 +74,49 @@ struct tls_runtime {
   }
 };
 
+template <class It>
+const void *address_of(It it) {
+  return static_cast<const void *>(std::addressof(*it));
+}
+
+// Whether a call of operation `op` over n elements of T touching the memory
+// of `its` runs on the GPU.  n > 0.
+template <class T, class... Its>
+bool offload(int op, std::size_t n, Its... its) {
+  const void *ptrs[] = {address_of(its)...};
+  int dec = 1;
+  acpp::detail::check_status(acpp_stdpar_should_offload(
+      tls_runtime::get().q.get_device(), op, static_cast<int>(sizeof(T)), std::is_floating_point<T>::value ? 1 : 0,
+      static_cast<uint64_t>(n), ptrs, static_cast<uint32_t>(sizeof...(Its)), &dec));
+  if (dec) ++tls_runtime::get().offloaded;
+  return dec != 0;
+}
+
+// The host side runs once the thread's offloaded work has completed.
+inline void before_host() { tls_runtime::get().q.wait(); }
+
+template <class It>
+using value_t = typename std::iterator_traits<It>::value_type;
+
 }  // namespace detail
+
+// Waits for this thread's offloaded work (the scans return without waiting)
+// and throws its asynchronous errors.
+inline void sync() { detail::tls_runtime::get().q.wait(); }
+
+// Number of this thread's calls that ran on the GPU so far.
+inline std::size_t offloaded_calls() { return detail::tls_runtime::get().offloaded; }
 
 // ---------------------------------------------------------------- reduce
 
 template <class ForwardIt1, class ForwardIt2, class T>
 T transform_reduce(par_unseq_policy, ForwardIt1 first1, ForwardIt1 last1, ForwardIt2 first2, T init) {
   if (first1 == last1) return init;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first1, last1));
+  if (!detail::offload<T>(ACPP_STDPAR_OP_BINARY_TRANSFORM_REDUCE, n, first1, first2)) {
+    detail::before_host();
+    return std::transform_reduce(first1, last1, first2, init);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group out_group{&rt.host_cache, rt.q};
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
@@ -69,6 +130,11 @@ template <class ForwardIt1, class ForwardIt2, class T, class BinaryReductionOp, 
 T transform_reduce(par_unseq_policy, ForwardIt1 first1, ForwardIt1 last1, ForwardIt2 first2, T init,
                    BinaryReductionOp reduce, BinaryTransformOp transform) {
   if (first1 == last1) return init;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first1, last1));
+  if (!detail::offload<T>(ACPP_STDPAR_OP_BINARY_TRANSFORM_REDUCE, n, first1, first2)) {
+    detail::before_host();
+    return std::transform_reduce(first1, last1, first2, init, reduce, transform);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group out_group{&rt.host_cache, rt.q};
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
@@ -82,6 +148,11 @@ template <class ForwardIt, class T, class BinaryReductionOp, class UnaryTransfor
 T transform_reduce(par_unseq_policy, ForwardIt first, ForwardIt last, T init, BinaryReductionOp reduce,
                    UnaryTransformOp transform) {
   if (first == last) return init;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<T>(ACPP_STDPAR_OP_TRANSFORM_REDUCE, n, first)) {
+    detail::before_host();
+    return std::transform_reduce(first, last, init, reduce, transform);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group out_group{&rt.host_cache, rt.q};
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
@@ -94,6 +165,11 @@ T transform_reduce(par_unseq_policy, ForwardIt first, ForwardIt last, T init, Bi
 template <class ForwardIt, class T, class BinaryOp>
 T reduce(par_unseq_policy, ForwardIt first, ForwardIt last, T init, BinaryOp op) {
   if (first == last) return init;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<T>(ACPP_STDPAR_OP_REDUCE, n, first)) {
+    detail::before_host();
+    return std::reduce(first, last, init, op);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group out_group{&rt.host_cache, rt.q};
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
@@ -105,27 +181,31 @@ T reduce(par_unseq_policy, ForwardIt first, ForwardIt last, T init, BinaryOp op)
 
 template <class ForwardIt, class T>
 T reduce(par_unseq_policy p, ForwardIt first, ForwardIt last, T init) {
-  return reduce(p, first, last, init, std::plus<T>{});
+  return ::acpp::stdpar::reduce(p, first, last, init, std::plus<T>{});
 }
 
 template <class ForwardIt>
-typename std::iterator_traits<ForwardIt>::value_type reduce(par_unseq_policy p, ForwardIt first,
-                                                            ForwardIt last) {
-  using T = typename std::iterator_traits<ForwardIt>::value_type;
-  return reduce(p, first, last, T{}, std::plus<T>{});
+detail::value_t<ForwardIt> reduce(par_unseq_policy p, ForwardIt first, ForwardIt last) {
+  using T = detail::value_t<ForwardIt>;
+  return ::acpp::stdpar::reduce(p, first, last, T{}, std::plus<T>{});
 }
 
 // ------------------------------------------------------------------ scans
+// Offloaded scans return once their work is enqueued; see sync().
 
 template <class InputIt, class OutputIt, class BinaryOp>
 OutputIt inclusive_scan(par_unseq_policy, InputIt first, InputIt last, OutputIt d_first, BinaryOp op) {
   OutputIt result = d_first;
   std::advance(result, std::distance(first, last));
   if (first == last) return result;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<detail::value_t<InputIt>>(ACPP_STDPAR_OP_INCLUSIVE_SCAN, n, first, d_first)) {
+    detail::before_host();
+    return std::inclusive_scan(first, last, d_first, op);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
   algorithms::inclusive_scan(rt.q, scratch, first, last, d_first, op);
-  rt.q.wait();
   return result;
 }
 
@@ -135,16 +215,20 @@ OutputIt inclusive_scan(par_unseq_policy, InputIt first, InputIt last, OutputIt 
   OutputIt result = d_first;
   std::advance(result, std::distance(first, last));
   if (first == last) return result;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<detail::value_t<InputIt>>(ACPP_STDPAR_OP_INCLUSIVE_SCAN, n, first, d_first)) {
+    detail::before_host();
+    return std::inclusive_scan(first, last, d_first, op, init);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
   algorithms::inclusive_scan(rt.q, scratch, first, last, d_first, op, init);
-  rt.q.wait();
   return result;
 }
 
 template <class InputIt, class OutputIt>
 OutputIt inclusive_scan(par_unseq_policy p, InputIt first, InputIt last, OutputIt d_first) {
-  return inclusive_scan(p, first, last, d_first, std::plus<>{});
+  return ::acpp::stdpar::inclusive_scan(p, first, last, d_first, std::plus<>{});
 }
 
 template <class InputIt, class OutputIt, class T, class BinaryOp>
@@ -153,16 +237,20 @@ OutputIt exclusive_scan(par_unseq_policy, InputIt first, InputIt last, OutputIt 
   OutputIt result = d_first;
   std::advance(result, std::distance(first, last));
   if (first == last) return result;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<detail::value_t<InputIt>>(ACPP_STDPAR_OP_EXCLUSIVE_SCAN, n, first, d_first)) {
+    detail::before_host();
+    return std::exclusive_scan(first, last, d_first, init, op);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
   algorithms::exclusive_scan(rt.q, scratch, first, last, d_first, init, op);
-  rt.q.wait();
   return result;
 }
 
 template <class InputIt, class OutputIt, class T>
 OutputIt exclusive_scan(par_unseq_policy p, InputIt first, InputIt last, OutputIt d_first, T init) {
-  return exclusive_scan(p, first, last, d_first, init, std::plus<>{});
+  return ::acpp::stdpar::exclusive_scan(p, first, last, d_first, init, std::plus<>{});
 }
 
 // pstl-impl/numeric.hpp:471-580.
@@ -172,10 +260,14 @@ OutputIt transform_inclusive_scan(par_unseq_policy, InputIt first, InputIt last,
   OutputIt result = d_first;
   std::advance(result, std::distance(first, last));
   if (first == last) return result;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<detail::value_t<InputIt>>(ACPP_STDPAR_OP_INCLUSIVE_SCAN, n, first, d_first)) {
+    detail::before_host();
+    return std::transform_inclusive_scan(first, last, d_first, binary_op, unary_op);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
   algorithms::transform_inclusive_scan(rt.q, scratch, first, last, d_first, binary_op, unary_op);
-  rt.q.wait();
   return result;
 }
 
@@ -185,10 +277,14 @@ OutputIt transform_inclusive_scan(par_unseq_policy, InputIt first, InputIt last,
   OutputIt result = d_first;
   std::advance(result, std::distance(first, last));
   if (first == last) return result;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<detail::value_t<InputIt>>(ACPP_STDPAR_OP_INCLUSIVE_SCAN, n, first, d_first)) {
+    detail::before_host();
+    return std::transform_inclusive_scan(first, last, d_first, binary_op, unary_op, init);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
   algorithms::transform_inclusive_scan(rt.q, scratch, first, last, d_first, binary_op, unary_op, init);
-  rt.q.wait();
   return result;
 }
 
@@ -198,10 +294,14 @@ OutputIt transform_exclusive_scan(par_unseq_policy, InputIt first, InputIt last,
   OutputIt result = d_first;
   std::advance(result, std::distance(first, last));
   if (first == last) return result;
+  const std::size_t n = static_cast<std::size_t>(std::distance(first, last));
+  if (!detail::offload<detail::value_t<InputIt>>(ACPP_STDPAR_OP_EXCLUSIVE_SCAN, n, first, d_first)) {
+    detail::before_host();
+    return std::transform_exclusive_scan(first, last, d_first, init, binary_op, unary_op);
+  }
   auto &rt = detail::tls_runtime::get();
   algorithms::util::allocation_group scratch{&rt.device_cache, rt.q};
   algorithms::transform_exclusive_scan(rt.q, scratch, first, last, d_first, init, binary_op, unary_op);
-  rt.q.wait();
   return result;
 }
 

@@ -343,3 +343,22 @@ def transform_reduce_binary(a: np.ndarray, b: np.ndarray, init, op: int, code: i
         c_void_p(_p(a)), c_void_p(_p(b)), c_uint64(a.size), _CT[t](init), c_int(op), c_int(code),
         ctypes.byref(out)))
     return a.dtype.type(out.value)
+
+
+# group data movement operation codes (oracle_group_move_<t>)
+MOVE_BROADCAST, MOVE_SHIFT_LEFT, MOVE_SHIFT_RIGHT, MOVE_PERMUTE_XOR, MOVE_SELECT, MOVE_ANY, MOVE_ALL, \
+    MOVE_NONE, MOVE_JOINT_ANY, MOVE_JOINT_ALL, MOVE_JOINT_NONE = range(11)
+
+
+def group_move(level: int, which: int, param: int, x: np.ndarray, src: np.ndarray | None,
+               local_size: int) -> np.ndarray:
+    """Group data movement (group_functions.hpp:57-140, 240-276) within
+    sub-groups of 64 (level 0) or work-groups (level 1)."""
+    x = np.ascontiguousarray(x)
+    t = _tag(x)
+    s = np.zeros(x.size, dtype=np.uint32) if src is None else np.ascontiguousarray(src, dtype=np.uint32)
+    out = np.empty_like(x)
+    _ok(getattr(load(), f"oracle_group_move_{t}")(c_int(level), c_int(which), ctypes.c_int64(param),
+                                                  c_void_p(_p(x)), c_void_p(_p(s)), c_uint64(x.size),
+                                                  c_uint64(local_size), c_void_p(_p(out))))
+    return out

@@ -793,3 +793,63 @@ DEF_BINARY_REDUCE(u32, uint32_t)
 DEF_BINARY_REDUCE(u64, uint64_t)
 DEF_BINARY_REDUCE(f32, float)
 DEF_BINARY_REDUCE(f64, double)
+
+/* ------------------------------------------------------------------------- */
+/* Group data movement (include/hipSYCL/sycl/libkernel/group_functions.hpp:   */
+/* 57-140 broadcast / any / all / none, 240-276 shift_group_left/right,       */
+/* permute_group_by_xor, select_from_group).  The elements are split into     */
+/* work-groups of local_size (the last may be partial); level 0 applies the   */
+/* operation within sub-groups of 64 consecutive work-items of a work-group   */
+/* (the wave64 sub-group), level 1 within the work-group.  which: 0 broadcast */
+/* of member param, 1 shift left by param, 2 shift right by param, 3 permute  */
+/* by xor with param, 4 select member src[i], 5 any (x > param), 6 all, 7     */
+/* none, 8-10 the joint_ forms of 5-7 (over the group's elements).  A source  */
+/* outside the group leaves the member's own value at sub-group level and     */
+/* gives member 0's value at work-group level (the reference leaves the       */
+/* result undefined there).  Fault 16 shifts right by param - 1.              */
+/* ------------------------------------------------------------------------- */
+#define DEF_GROUP_MOVE(t, T)                                                                          \
+  int oracle_group_move_##t(int level, int which, int64_t param, const T *x, const uint32_t *src,    \
+                            uint64_t n, uint64_t local_size, T *out) {                               \
+    if (local_size == 0 || which < 0 || which > 10 || level < 0 || level > 1) return -1;             \
+    for (uint64_t b0 = 0; b0 < n; b0 += local_size) {                                                \
+      const uint64_t wg = (b0 + local_size < n) ? local_size : n - b0;                               \
+      const uint64_t span = level == 0 ? 64 : wg;                                                     \
+      for (uint64_t g0 = 0; g0 < wg; g0 += span) {                                                    \
+        const uint64_t size = (g0 + span < wg) ? span : wg - g0;                                      \
+        const T *xs = x + b0 + g0;                                                                    \
+        int any = 0, all = 1;                                                                         \
+        for (uint64_t l = 0; l < size; ++l) {                                                         \
+          const int p = (double)xs[l] > (double)param;                                                \
+          any |= p;                                                                                   \
+          all &= p;                                                                                   \
+        }                                                                                             \
+        for (uint64_t l = 0; l < size; ++l) {                                                         \
+          const uint64_t i = b0 + g0 + l;                                                             \
+          const T own = level == 0 ? xs[l] : xs[0];                                                   \
+          int64_t s = -1;                                                                             \
+          switch (which) {                                                                            \
+            case 0: s = param; break;                                                                 \
+            case 1: s = (int64_t)l + param; break;                                                    \
+            case 2: s = (int64_t)l - (g_fault == 16 ? param - 1 : param); break;                      \
+            case 3: s = (int64_t)(l ^ (uint64_t)param); break;                                        \
+            case 4: s = (int64_t)src[i]; break;                                                       \
+            default: break;                                                                           \
+          }                                                                                           \
+          if (which <= 4)                                                                             \
+            out[i] = (s >= 0 && (uint64_t)s < size) ? xs[s] : own;                                   \
+          else if (which == 5 || which == 8)                                                          \
+            out[i] = (T)(any ? 1 : 0);                                                                \
+          else if (which == 6 || which == 9)                                                          \
+            out[i] = (T)(all ? 1 : 0);                                                                \
+          else                                                                                        \
+            out[i] = (T)(any ? 0 : 1);                                                                \
+        }                                                                                             \
+      }                                                                                               \
+    }                                                                                                 \
+    return 0;                                                                                         \
+  }
+DEF_GROUP_MOVE(i32, int32_t)
+DEF_GROUP_MOVE(i64, int64_t)
+DEF_GROUP_MOVE(f32, float)
+DEF_GROUP_MOVE(f64, double)

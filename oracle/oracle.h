@@ -103,7 +103,7 @@ int oracle_num_procs(void);
    restatements: 8 sixteen elements per work-item, 9 serial left fold, 10
    adjacent-pairs tree, 11 swapped tree operands, 12 1024 x 8 scan tiles, 13
    swapped work-item fold operands, 14 op(result, init); 15 |x-y| computed
-   as x-y in the binary transform_reduce. */
+   as x-y in the binary transform_reduce; 16 shift_group_right by param - 1. */
 void oracle_set_fault(int fault);
 
 /* Element transforms (transform codes 0..7 of acpp_unary_op); -1 for an
@@ -128,6 +128,14 @@ int oracle_transform_reduce_binary_u32(const uint32_t *a, const uint32_t *b, uin
 int oracle_transform_reduce_binary_u64(const uint64_t *a, const uint64_t *b, uint64_t n, uint64_t init, int op, int code, uint64_t *out);
 int oracle_transform_reduce_binary_f32(const float *a, const float *b, uint64_t n, float init, int op, int code, float *out);
 int oracle_transform_reduce_binary_f64(const double *a, const double *b, uint64_t n, double init, int op, int code, double *out);
+
+/* Group data movement (broadcast, shifts, permute by xor, select, any /
+   all / none) within sub-groups of 64 (level 0) or work-groups (level 1) of
+   local_size elements; see oracle.c for the operation codes. */
+int oracle_group_move_i32(int level, int which, int64_t param, const int32_t *x, const uint32_t *src, uint64_t n, uint64_t local_size, int32_t *out);
+int oracle_group_move_i64(int level, int which, int64_t param, const int64_t *x, const uint32_t *src, uint64_t n, uint64_t local_size, int64_t *out);
+int oracle_group_move_f32(int level, int which, int64_t param, const float *x, const uint32_t *src, uint64_t n, uint64_t local_size, float *out);
+int oracle_group_move_f64(int level, int which, int64_t param, const double *x, const uint32_t *src, uint64_t n, uint64_t local_size, double *out);
 
 #ifdef __cplusplus
 }

@@ -176,3 +176,37 @@ def test_reduction_and_copy_if_on_gpu():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL PASSED" in r.stdout
+
+
+STDPAR_STD_BIN = os.path.join(ROOT, "tests", "lib", "test_stdpar_std")
+
+
+def test_stdpar_std_overloads_chosen_over_the_standard_library(tmp_path):
+    # with include/acpp/stdpar/std first, std::reduce(par_unseq, ...) must
+    # resolve to the forwarding overload (libstdc++'s own templates on any
+    # execution policy would make the call ambiguous or bypass the library)
+    src = tmp_path / "s.cpp"
+    src.write_text("""
+#include <numeric>
+#include <execution>
+#include <vector>
+long f(std::vector<long>& v, std::vector<long>& w) {
+  std::inclusive_scan(std::execution::par_unseq, v.begin(), v.end(), w.begin());
+  std::exclusive_scan(std::execution::par_unseq, v.begin(), v.end(), w.begin(), 3L);
+  return std::reduce(std::execution::par_unseq, v.begin(), v.end()) +
+         std::transform_reduce(std::execution::par_unseq, v.begin(), v.end(), w.begin(), 0L) +
+         std::reduce(std::execution::seq, v.begin(), v.end());
+}
+""")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{os.path.join(INCLUDE, 'acpp', 'stdpar', 'std')}",
+                        f"-I{INCLUDE}", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_stdpar_std_on_gpu():
+    r = subprocess.run([STDPAR_STD_BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout

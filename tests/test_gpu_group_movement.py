@@ -1,8 +1,11 @@
 """GPU parity: group_broadcast, any/all/none_of_group, joint_*_of,
 shift_group_left/right, permute_group_by_xor and select_from_group from
 include/acpp/device/group_functions.hpp, for sub-groups (wave64) and
-work-groups, against a numpy model of the SYCL 2020 semantics
-(include/hipSYCL/sycl/libkernel/group_functions.hpp:57-140, 240-276).
+work-groups, against the oracle's restatement of the SYCL 2020 semantics
+(oracle_group_move_<t>; include/hipSYCL/sycl/libkernel/group_functions.hpp:
+57-140, 240-276), itself pinned by the cases of the reference's
+tests/sycl/group_functions/group_functions_misc.cpp
+(tests/test_oracle_group_movement.py).
 
 Out-of-range sources: a sub-group lane keeps its own value (the HIP
 shuffles); a work-group reads work-item 0 (the reference's LDS fallback,
@@ -13,6 +16,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle as O
 from gpu_util import dev, host
 
 pytestmark = pytest.mark.gpu
@@ -30,42 +34,6 @@ def tk():
     lib.tk_group_movement.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, vp, vp, vp,
                                       ctypes.c_uint64, ctypes.c_uint32, vp]
     return lib
-
-
-def model(level, which, param, x, src, local_size):
-    n = x.size
-    out = np.empty_like(x)
-    for b0 in range(0, n, local_size):
-        if level == 0:
-            groups = [(b0 + w, min(64, local_size - (w))) for w in range(0, local_size, 64)]
-        else:
-            groups = [(b0, local_size)]
-        for g0, size in groups:
-            xs = x[g0:g0 + size]
-            pred = xs > param
-            for l in range(size):
-                i = g0 + l
-                own = xs[l] if level == 0 else xs[0]
-
-                def fetch(s):
-                    return xs[s] if 0 <= s < size else own
-                if which == 0:
-                    out[i] = xs[param]
-                elif which == 1:
-                    out[i] = fetch(l + param)
-                elif which == 2:
-                    out[i] = fetch(l - param) if l >= param else own
-                elif which == 3:
-                    out[i] = fetch(l ^ param)
-                elif which == 4:
-                    out[i] = fetch(int(src[i]))
-                elif which in (5, 8):
-                    out[i] = 1 if pred.any() else 0
-                elif which in (6, 9):
-                    out[i] = 1 if pred.all() else 0
-                else:
-                    out[i] = 0 if pred.any() else 1
-    return out
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64, np.float32, np.float64])
@@ -97,5 +65,5 @@ def test_movement(gpu, tk, dtype, local_size, level):
             thr = np.array(pv & 0xffffffff, dtype=np.uint32).astype(np.int64)
             thr = thr if thr < 2**31 else thr - 2**32
             got = host(od)
-            exp = model(level, which, thr if which >= 5 else pv, x, src, local_size)
+            exp = O.group_move(level, which, int(thr if which >= 5 else pv), x, src, local_size)
             assert np.array_equal(got, exp), (name, param, local_size, level)
