@@ -366,7 +366,35 @@ def segmented_leg(acpp, q, cache, per_segment=True):
         plan.close()
     del xs, ys, yref, xf
     torch.cuda.empty_cache()
+    out["learned_tuner"] = tuner_report(tuner)
     log(f"segmented: {out}")
+    return out
+
+
+def tuner_report(tuner):
+    """The learned configuration choice on the 65,536-segment workload (int64
+    scan, fp32 reduce): its pick, the library's default, and the best of all
+    15 configurations timed afresh here (exhaustive autotune)."""
+    import torch
+    out = {}
+    learned = tuner.LearnedTuner()
+    for kind, dtype in (("scan", "i64"), ("reduce", "f32")):
+        shape = tuner.lognormal_workload(kind, dtype, S=65536, seed=0)
+        pick = learned.choose(shape)
+        times = tuner.autotune(shape, reps=5)
+        best = min(times, key=times.get)
+        dflt = tuner.default_config(kind, shape.n)
+        key = lambda c: f"{c[0]}x{c[1]}"
+        out[f"{kind}_{dtype}"] = {
+            "learned_pick": key(pick), "learned_us": round(times[pick], 1),
+            "best": key(best), "best_us": round(times[best], 1),
+            "learned_regret": round(times[pick] / times[best] - 1.0, 4),
+            "default": key(dflt), "default_us": round(times[dflt], 1),
+            "default_regret": round(times[dflt] / times[best] - 1.0, 4),
+            "configs_timed": len(times),
+        }
+        torch.cuda.empty_cache()
+    log(f"learned tuner: {out}")
     return out
 
 

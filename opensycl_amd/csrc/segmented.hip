@@ -292,16 +292,68 @@ __device__ __forceinline__ uint64_t find_segment(const uint64_t *offsets, uint64
   return lo;
 }
 
+// threadIdx.x through an empty asm statement: a value the compiler cannot
+// move, so what is derived from it inside a loop stays inside the loop.
+__device__ __forceinline__ int loop_local_thread_index() {
+  int t = static_cast<int>(threadIdx.x);
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// Head bits of a work-item's R row vectors from the tile's LDS bitmap, all in
+// one word: bit r * VEC + k = element k of row r starts a segment.  VEC
+// consecutive bits never straddle a bitmap word (VEC divides 32).
+template <int R, int VEC>
+__device__ __forceinline__ uint32_t row_head_bits(const uint32_t *s_heads, int tid, int B) {
+  static_assert(R * VEC <= 32, "head bits of a work-item must fit one word");
+  uint32_t fm = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t pos = static_cast<uint32_t>(r * B + tid) * VEC;
+    fm |= ((s_heads[pos >> 5] >> (pos & 31)) & ((1u << VEC) - 1u)) << (r * VEC);
+  }
+  return fm;
+}
+
+// Wave scans of the (head flag, total) pairs of each row, after the
+// work-item-local scans.  Lane 63 leaves the wave's row total in
+// s_tf / s_tv[r * nw + w]; every work-item folds its wave-exclusive prefix
+// into the row's elements up to the row's first segment start and returns
+// bit r set when a segment starts in an earlier lane of row r (the elements
+// then hold their full in-segment fold already).  Folding at once keeps one
+// bit per row live instead of a (flag, value) pair.
+template <class Op, int R, int VEC, class W>
+__device__ __forceinline__ uint32_t fold_wave_prefixes(uint32_t fm, W (&v)[R][VEC], uint32_t *s_tf, W *s_tv,
+                                                       int nw, int w, int lane) {
+  const W id = Op::identity();
+  uint32_t xcut = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t rowf = (fm >> (r * VEC)) & ((1u << VEC) - 1u);
+    uint32_t tf = rowf != 0u;
+    W tv = v[r][VEC - 1];
+    wave_pair_scan<Op>(tf, tv);
+    if (lane == 63) {
+      s_tf[r * nw + w] = tf;
+      s_tv[r * nw + w] = tv;
+    }
+    const uint32_t ef = wave::dpp<wave::wave_shr1, 0xf, 0xf>(0u, tf);
+    const W ev = wave::dpp<wave::wave_shr1, 0xf, 0xf>(id, tv);
+    xcut |= ef << r;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k)
+      if ((rowf & ((2u << k) - 1u)) == 0u) v[r][k] = Op::apply(ev, v[r][k]);
+  }
+  return xcut;
+}
+
 template <class T, class Op, int R>
 __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
   using W = typename elem_traits<T>::work;
   using SIO = seg_status<W>;
   constexpr int VEC = 16 / sizeof(T);
-  const int tid = threadIdx.x;
   const int B = blockDim.x;
   const int nw = B >> 6;
-  const int w = tid >> 6;
-  const int lane = tid & 63;
   const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
   const W id = Op::identity();
 
@@ -324,6 +376,13 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
   // chain (drawing one tile ahead made the 65,536-segment int64 scan ~3x
   // slower on MI355X).
   for (;;) {
+    // The work-item's indices are re-derived in every iteration, so that the
+    // per-row addresses and bit positions computed from them are too: hoisted
+    // out of the loop they held ~60 VGPRs for the whole kernel and made
+    // 8-row tiles spill.
+    const int tid = loop_local_thread_index();
+    const int w = tid >> 6;
+    const int lane = tid & 63;
     __syncthreads();  // LDS of the previous tile is free again
     if (tid == 0)
       s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED,
@@ -354,6 +413,15 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
       const u32x4_a4 *src = reinterpret_cast<const u32x4_a4 *>(p.in + base);
 #pragma unroll
       for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+      // Converted at once: the copies only rename registers (4- and 8-byte
+      // elements), and the loaded vectors then never live beside v.
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T tmp[VEC];
+        __builtin_memcpy(tmp, &q[r], 16);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+      }
     } else {
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -366,29 +434,14 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
     if (static_cast<uint32_t>(tid) < head_words) s_heads[tid] = hword0;
     if (static_cast<uint32_t>(tid + B) < head_words) s_heads[tid + B] = hword1;
     __syncthreads();
-    if (full) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        T tmp[VEC];
-        __builtin_memcpy(tmp, &q[r], 16);
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
-      }
-    }
 
     auto head_at = [&](uint64_t pos) -> uint32_t {
       return (s_heads[pos >> 5] >> (pos & 31)) & 1u;
     };
 
-    // fm[r] bit k: element k of row r starts a segment.
-    uint32_t fm[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      // VEC consecutive bits of the bitmap (VEC divides 32, so they never
-      // straddle a word).
-      const uint64_t pos = static_cast<uint64_t>(r * B + tid) * VEC;
-      fm[r] = (s_heads[pos >> 5] >> (pos & 31)) & ((1u << VEC) - 1u);
-    }
+    // Head bits of the work-item's elements, all rows in one word: bit
+    // r * VEC + k = element k of row r starts a segment.
+    const uint32_t fm = row_head_bits<R, VEC>(s_heads, tid, B);
 
     // Work-item-local segmented scan of each row vector: an element that
     // starts a segment keeps its value, the others add the running value.
@@ -396,23 +449,13 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
     for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int k = 1; k < VEC; ++k)
-        v[r][k] = ((fm[r] >> k) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
+        v[r][k] = ((fm >> (r * VEC + k)) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
 
-    // Wave scans of the row totals.
-    uint32_t xf[R];
-    W xv[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      uint32_t tf = fm[r] != 0u;
-      W tv = v[r][VEC - 1];
-      wave_pair_scan<Op>(tf, tv);
-      if (lane == 63) {
-        s_tf[r * nw + w] = tf;
-        s_tv[r * nw + w] = tv;
-      }
-      xf[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(0u, tf);
-      xv[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(id, tv);
-    }
+    // Wave scans of the row totals; the wave-exclusive prefix is folded into
+    // the row's elements at once (up to the row's first segment start), so
+    // only one bit per row stays live: bit r of xcut = a segment starts in
+    // an earlier lane of this wave's row r.
+    const uint32_t xcut = fold_wave_prefixes<Op, R, VEC>(fm, v, s_tf, s_tv, nw, w, lane);
     __syncthreads();
     // Wave 0 turns the (row, wave) totals into exclusive prefixes and
     // publishes the tile's aggregate at once.  A tile that contains a
@@ -433,14 +476,9 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
       }
     }
     __syncthreads();
-    // Pairs before (row r, wave w) inside the tile, in element order.
-    uint32_t bf[R];
-    W bv[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      bf[r] = s_tf[r * nw + w];
-      bv[r] = s_tv[r * nw + w];
-    }
+    // s_tf / s_tv now hold the pairs before each (row, wave) inside the
+    // tile, in element order; they are read again for the results (the
+    // next tile rewrites them only after the barrier at the top of the loop).
     const uint32_t agg_f = s_aggf;
     const W agg_v = s_aggv;
 
@@ -560,34 +598,24 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
       }
     }
 
-    // Results.
+    // Results: the elements already hold the fold from the start of their
+    // wave's row (or of their segment); the rest is P = tile prefix . before
+    // (row r, wave w), cut off by a segment start in an earlier lane or
+    // among elements 0..k of the vector.
+    (void)pf;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      // P = tile prefix . before . wave-exclusive
-      uint32_t Pf = bf[r];
-      W Pv = bv[r];
-      {
-        uint32_t tf = xf[r];
-        W tv = xv[r];
-        combine<Op>(Pf, Pv, tf, tv);
-        Pf = tf;
-        Pv = tv;
-      }
-      {
-        uint32_t tf = Pf;
-        W tv = Pv;
-        combine<Op>(pf, pv, tf, tv);
-        Pf = tf;
-        Pv = tv;
-      }
+      const uint32_t bfr = s_tf[r * nw + w];
+      const W bvr = s_tv[r * nw + w];
+      const W P = bfr ? bvr : Op::apply(pv, bvr);
+      const uint32_t rowf = (fm >> (r * VEC)) & ((1u << VEC) - 1u);
+      const bool wcut = (xcut >> r) & 1u;
       W o[VEC];
 #pragma unroll
       for (int k = 0; k < VEC; ++k) {
-        // a segment start among elements 0..k of the vector cuts the prefix off
-        const bool cut = (fm[r] & ((2u << k) - 1u)) != 0u;
-        o[k] = cut ? v[r][k] : Op::apply(Pv, v[r][k]);
+        const bool cut = wcut || (rowf & ((2u << k) - 1u)) != 0u;
+        o[k] = cut ? v[r][k] : Op::apply(P, v[r][k]);
       }
-      (void)Pf;
       const uint64_t e0 = base + static_cast<uint64_t>(r * B + tid) * VEC;
       if (full) {
         T ot[VEC];
@@ -714,32 +742,14 @@ __global__ __launch_bounds__(1024) void segmented_reduce_kernel(params<T> p,
     return s_lo + lo;
   };
 
-  uint32_t fm[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint64_t pos = static_cast<uint64_t>(r * B + tid) * VEC;
-    fm[r] = (s_heads[pos >> 5] >> (pos & 31)) & ((1u << VEC) - 1u);
-  }
+  const uint32_t fm = row_head_bits<R, VEC>(s_heads, tid, B);
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int k = 1; k < VEC; ++k)
-      v[r][k] = ((fm[r] >> k) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
+      v[r][k] = ((fm >> (r * VEC + k)) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
 
-  uint32_t xf[R];
-  W xv[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    uint32_t tf = fm[r] != 0u;
-    W tv = v[r][VEC - 1];
-    wave_pair_scan<Op>(tf, tv);
-    if (lane == 63) {
-      s_tf[r * nw + w] = tf;
-      s_tv[r * nw + w] = tv;
-    }
-    xf[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(0u, tf);
-    xv[r] = wave::dpp<wave::wave_shr1, 0xf, 0xf>(id, tv);
-  }
+  const uint32_t xcut = fold_wave_prefixes<Op, R, VEC>(fm, v, s_tf, s_tv, nw, w, lane);
   __syncthreads();
   if (w == 0) {
     uint32_t af;
@@ -752,20 +762,17 @@ __global__ __launch_bounds__(1024) void segmented_reduce_kernel(params<T> p,
   const uint64_t tile_left = p.total - base;  // > 0: the tile holds elements
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    // (row r, wave w) prefix inside the tile . wave-exclusive prefix
-    uint32_t Pf = s_tf[r * nw + w];
-    W Pv = s_tv[r * nw + w];
-    {
-      uint32_t tf = xf[r];
-      W tv = xv[r];
-      combine<Op>(Pf, Pv, tf, tv);
-      Pf = tf;
-      Pv = tv;
-    }
+    // The elements hold the fold from their wave's row start (or segment
+    // start); Pv is the (row r, wave w) prefix inside the tile, cut off by
+    // a segment start in an earlier lane.
+    const uint32_t rowf = (fm >> (r * VEC)) & ((1u << VEC) - 1u);
+    const bool wcut = (xcut >> r) & 1u;
+    const uint32_t Pf = s_tf[r * nw + w] | (wcut ? 1u : 0u);
+    const W Pv = s_tv[r * nw + w];
     const uint64_t l0 = static_cast<uint64_t>(r * B + tid) * VEC;  // index inside the tile
     // Bit k: element k of the vector is the last of its segment in this
     // tile (a segment start follows, or the data or the tile ends there).
-    uint32_t ends = (fm[r] >> 1) | (head_at(l0 + VEC) << (VEC - 1));
+    uint32_t ends = (rowf >> 1) | (head_at(l0 + VEC) << (VEC - 1));
     if (l0 + VEC == tile_elems) ends |= 1u << (VEC - 1);
     if (l0 < tile_left && tile_left <= l0 + VEC) ends |= 1u << (tile_left - 1 - l0);
     if (tile_left <= l0) ends = 0;
@@ -776,7 +783,7 @@ __global__ __launch_bounds__(1024) void segmented_reduce_kernel(params<T> p,
       if (!((ends >> k) & 1u)) continue;
       const uint64_t e = base + l0 + k;
       // a segment start among elements 0..k of the vector cuts the prefix off
-      const bool cut = (fm[r] & ((2u << k) - 1u)) != 0u;
+      const bool cut = wcut || (rowf & ((2u << k) - 1u)) != 0u;
       const W val = cut ? v[r][k] : Op::apply(Pv, v[r][k]);
       const bool next_head = (e + 1 == p.total) || head_at(l0 + k + 1);
       if (cut || Pf) {  // a segment start lies in [tile start, e]
@@ -827,22 +834,26 @@ __global__ void segmented_fixup_kernel(params<T> p, const typename elem_traits<T
 // Host side
 // --------------------------------------------------------------------------
 // Default tile when the caller gives no configuration, by total element
-// count.  Measured on MI355X over 500 random segmented shapes (the tuner's
-// timing database): this rule has a mean regret of ~5 % against the best of
-// the 15 configurations, where any single fixed tile has 10-30 %.
-//   scan:   n < 2^16: 256 x 2;  n < 2^20: 1024 x 2;  otherwise 1024 x 4
+// count.  Measured on MI355X over 500 segmented shapes up to 2^28 elements
+// (the tuner's timing database): this rule has a mean regret of ~7 %
+// against the best of the 15 configurations, where any single fixed tile
+// has 15-70 %.
+//   scan:   n < 2^16: 256 x 2;  n < 2^20: 512 x 2;  otherwise 512 x 8
 //   reduce: n < 2^16: 128 x 2;  n < 2^20: 256 x 2;  n < 2^24: 1024 x 2;
-//           otherwise 1024 x 4
-// Eight rows spill and are only used when a caller asks for them.
+//           otherwise 256 x 4
+// Large scans take 8-row tiles, two 512-wide groups per CU (126 VGPRs for
+// int64); large reductions take 4-row tiles, whose 60 VGPRs leave room for
+// several groups per CU (the reduction's groups never wait for each other).
 inline uint32_t default_work_group(bool reduce, uint64_t total) {
   if (total < (1ull << 16)) return reduce ? 128 : 256;
-  if (total < (1ull << 20)) return reduce ? 256 : 1024;
-  return 1024;
+  if (total < (1ull << 20)) return reduce ? 256 : 512;
+  if (reduce && total < (1ull << 24)) return 1024;
+  return reduce ? 256 : 512;
 }
 inline int default_rows(bool reduce, uint64_t total) {
   if (total < (1ull << 20)) return 2;
-  if (reduce && total < (1ull << 24)) return 2;
-  return 4;
+  if (reduce) return total < (1ull << 24) ? 2 : 4;
+  return 8;
 }
 
 template <class T, class Op>

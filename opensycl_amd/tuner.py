@@ -56,10 +56,10 @@ def default_config(kind: str, n: int) -> tuple:
     if n < (1 << 16):
         return (128, 2) if kind == "reduce" else (256, 2)
     if n < (1 << 20):
-        return (256, 2) if kind == "reduce" else (1024, 2)
-    if kind == "reduce" and n < (1 << 24):
-        return (1024, 2)
-    return (1024, 4)
+        return (256, 2) if kind == "reduce" else (512, 2)
+    if kind == "reduce":
+        return (1024, 2) if n < (1 << 24) else (256, 4)
+    return (512, 8)
 
 KINDS = ("scan", "reduce")
 DTYPES = {"i32": 4, "i64": 8, "f32": 4, "f64": 8}
@@ -146,7 +146,7 @@ def features(shape, cfg) -> np.ndarray:
     return f
 
 
-def sample_shape(rng: np.random.Generator, max_elems: int = 1 << 25) -> Shape:
+def sample_shape(rng: np.random.Generator, max_elems: int = 1 << 28) -> Shape:
     """A random segmented workload: log-uniform segment count and median
     length, log-normal spread, clipped to [0 or 1, 2^20] per segment."""
     kind = KINDS[rng.integers(0, 2)]
@@ -256,13 +256,16 @@ def _unkey(k: str):
 
 def collect(num_shapes: int, seed: int = 0, reps: int = 5, path: Optional[str] = None,
             device: int = 0, progress: bool = True) -> list:
-    """Timing database: one record per random shape with every config's
-    time.  Written to `path` (JSON) as it grows."""
+    """Timing database: the 65,536-segment log-normal workload for every
+    kind and element type, then random shapes up to 2^28 elements, one
+    record per shape with every config's time.  Written to `path` (JSON) as
+    it grows."""
     rng = np.random.default_rng(seed)
+    fixed = [lognormal_workload(k, d) for k in KINDS for d in DTYPES]
     db = []
     t0 = time.time()
     for i in range(num_shapes):
-        shape = sample_shape(rng)
+        shape = fixed[i] if i < len(fixed) else sample_shape(rng)
         times = autotune(shape, reps=reps, device=device, seed=seed * 100003 + i)
         db.append({"shape": asdict(summarize(shape)), "times_us": {_key(c): t for c, t in times.items()}})
         if progress and (i + 1) % 10 == 0:

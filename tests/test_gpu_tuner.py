@@ -31,3 +31,25 @@ def test_autotune_times_every_config(gpu):
     times = T.autotune(s, reps=3)
     assert set(times) == set(T.CONFIGS)
     assert all(t > 0 for t in times.values())
+
+
+def test_learned_choice_regret_on_fresh_exhaustive_timings(gpu):
+    # Shapes the committed model never saw (a seed of their own), each timed
+    # afresh over all 15 configurations on this GPU: the learned pick's time
+    # against the best of them.  Mean regret at most 10 %.
+    learned = T.LearnedTuner()
+    rng = np.random.default_rng(20261203)
+    shapes = [T.sample_shape(rng, max_elems=1 << 26) for _ in range(12)]
+    picks = learned.choose_many(shapes)
+    regrets = []
+    for i, (s, pick) in enumerate(zip(shapes, picks)):
+        print(f"shape {i}: {s.kind} {s.dtype} {s.op} n={s.n} segments={s.num_segments}", flush=True)
+        t = T.autotune(s, reps=5, seed=i)
+        best = min(t, key=t.get)
+        regrets.append(t[pick] / t[best] - 1.0)
+        print(f"  pick {pick} {t[pick]:.1f} us, best {best} {t[best]:.1f} us, regret {regrets[-1]:.3f}",
+              flush=True)
+    mean = float(np.mean(regrets))
+    print(f"mean regret {mean:.4f} over {len(regrets)} shapes x {len(T.CONFIGS)} configurations")
+    assert len(regrets) >= 10 and len(T.CONFIGS) == 15
+    assert mean <= 0.10, regrets

@@ -36,7 +36,7 @@ def test_sample_shapes_bounded():
     rng = np.random.default_rng(0)
     for _ in range(200):
         s = T.sample_shape(rng)
-        assert 1 <= s.n <= 1 << 25
+        assert 1 <= s.n <= 1 << 28
         assert s.num_segments >= 1
         assert max(s.lengths) <= 1 << 20
         off = s.offsets()
@@ -108,5 +108,6 @@ def test_library_default_rule_on_committed_timings():
     r = T.regret(held, picks)
     assert r["mean_regret"] < 0.10, r
     assert r["mean_regret"] < r["fixed_512x4_mean_regret"], r
-    assert T.default_config("scan", 1 << 28) == (1024, 4)
+    assert T.default_config("scan", 1 << 28) == (512, 8)
+    assert T.default_config("reduce", 1 << 28) == (256, 4)
     assert T.default_config("reduce", 1000) == (128, 2)
