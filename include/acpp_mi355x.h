@@ -125,7 +125,10 @@ typedef struct acpp_launch_config {
                                 with the whole work-group (window = group
                                 size) instead of one wave (window 64);
                                 ACPP_VARIANT_NONTEMPORAL (reduce) streams the
-                                input with non-temporal loads;
+                                input with non-temporal loads, (scans of
+                                4- and 8-byte elements with operator plus)
+                                full tiles with non-temporal loads and
+                                stores;
                                 ACPP_VARIANT_TILE_PER_GROUP (segmented) one
                                 work-group per tile instead of a persistent
                                 grid */

@@ -43,7 +43,6 @@ struct params {
   T operand;
   T mask;
   int vec_io;
-  int out_vec;  // output 16-byte aligned
   int stencil_vec;  // stencil aligned to VEC bytes: one load per 16-byte data vector
   uint32_t num_tiles;
   uint32_t epoch;
@@ -214,32 +213,16 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
   }
   __syncthreads();
   // Store the tile's output range from LDS with consecutive work-items on
-  // consecutive elements (fully coalesced whatever the selection pattern).
+  // consecutive elements: coalesced whatever the selection pattern, and
+  // conflict-free in LDS.  Rows of 16-byte stores read the compacted tile
+  // with a stride of four words between lanes instead; half of copy_if's
+  // LDS cycles were then bank conflicts, and the 2^28 int32 compaction took
+  // 369 us against 357 us this way.
   const uint64_t tile_prefix = s_excl;
   if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + agg;
   T *dst = p.out + tile_prefix;
   const uint32_t count = static_cast<uint32_t>(agg);
-  if (!p.out_vec) {
-    for (uint32_t i = tid; i < count; i += B) dst[i] = s_stage[i];
-    return;
-  }
-  // 16-byte stores for the part of the range that is 16-byte aligned in the
-  // output; the elements before and after it are stored one by one.
-  const uint32_t mis = static_cast<uint32_t>(tile_prefix % VEC);
-  uint32_t head = mis ? static_cast<uint32_t>(VEC) - mis : 0u;
-  if (head > count) head = count;
-  const uint32_t nvec = (count - head) / VEC;
-  const uint32_t tail = head + nvec * VEC;
-  if (static_cast<uint32_t>(tid) < head) dst[tid] = s_stage[tid];
-  for (uint32_t j = tid; j < nvec; j += B) {
-    T t[VEC];
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) t[k] = s_stage[head + j * VEC + k];
-    u32x4 q;
-    __builtin_memcpy(&q, t, 16);
-    reinterpret_cast<u32x4 *>(dst + head)[j] = q;
-  }
-  if (static_cast<uint32_t>(tid) < count - tail) dst[tail + tid] = s_stage[tail + tid];
+  for (uint32_t i = tid; i < count; i += B) dst[i] = s_stage[i];
 }
 
 __global__ void store_zero_kernel(uint64_t *p) { *p = 0; }
@@ -305,7 +288,6 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   // 16-byte input loads need only element alignment for 4- and 8-byte
   // elements (see scan_detail::u32x4_a4).
   p.vec_io = (reinterpret_cast<uintptr_t>(in) & 3) == 0;
-  p.out_vec = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   p.num_tiles = static_cast<uint32_t>(num_tiles);
   p.epoch = rl.epoch;
   p.ticket_base = rl.ticket_base;

@@ -319,7 +319,7 @@ struct params {
 
 // XF: apply the element transform p.unary to every input element (the
 // transform scans); false for the plain scans.
-// NT: full tiles are read with non-temporal loads.
+// NT: full tiles are read and written with non-temporal loads and stores.
 template <class T, class Op, int R, bool XF = false, bool NT = false>
 __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
   using W = typename elem_traits<T>::work;
@@ -493,9 +493,13 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
     if (full) {
       u32x4 q;
       __builtin_memcpy(&q, o, 16);
-      // An ordinary store: non-temporal output stores measured no faster
-      // (380 us against 377 us at 2^28 int32).
-      reinterpret_cast<u32x4_a4 *>(p.out + base)[r * B + tid] = q;
+      // Ordinary stores by default: non-temporal output stores alone
+      // measured no faster (380 us against 377 us at 2^28 int32).  The NT
+      // variant streams both directions, as the fastest tile copy does.
+      if constexpr (NT)
+        __builtin_nontemporal_store(q, reinterpret_cast<u32x4_a4 *>(p.out + base) + r * B + tid);
+      else
+        reinterpret_cast<u32x4_a4 *>(p.out + base)[r * B + tid] = q;
     } else {
 #pragma unroll
       for (int k = 0; k < VEC; ++k) {

@@ -83,6 +83,10 @@ def collect(log2_sizes=None, reps: int = 5, log=print) -> list[dict]:
     are included so that the model sees sizes between powers of two."""
     if log2_sizes is None:
         log2_sizes = [x / 2 for x in range(8, 53)]  # 2^4 .. 2^26
+    # torch's HIP runtime must be initialised before the sampler's: the other
+    # order leaves torch without a device in this process.
+    import torch
+    torch.cuda.init()
     rows = []
     t0 = time.time()
     for op in OPS:
@@ -172,6 +176,9 @@ def save(model, rows: list[dict], extra: dict | None = None) -> None:
 def main() -> None:
     """Collect samples, train, report held-out regret, save (GPU box)."""
     rows = collect()
+    with open(SAMPLES_PATH + ".tmp", "w") as f:
+        json.dump(rows, f)
+    os.replace(SAMPLES_PATH + ".tmp", SAMPLES_PATH)
     tr, te = held_out(rows)
     model = train(tr)
     dec = decisions(model, te)

@@ -77,6 +77,25 @@ def test_scan_launch_configs(gpu, q, scratch, cfg):
     assert scratch.watchdog_trips() == 0
 
 
+# 1-byte elements: 4-row tiles widened in registers, at several group
+# sizes, every operator, with partial last tiles.
+@pytest.mark.parametrize("cfg", [None, (1024, 4), (512, 4), (256, 4), (64, 4)])
+def test_byte_scan_kernels(gpu, q, scratch, cfg):
+    torch, _ = gpu
+    for dtype in (np.int8, np.uint8):
+        for op in ALL_OPS:
+            for n in (1000, 3 * 131072 + 77):
+                x = op_input(dtype, op, n, seed=n + op)
+                init = dtype(3)
+                got = _scan_case(torch, q, scratch, x, op, "inclusive", config=cfg)
+                assert np.array_equal(got, O.inclusive_scan(x, op)), (dtype, op, n, cfg, "incl")
+                got = _scan_case(torch, q, scratch, x, op, "inclusive", init, config=cfg)
+                assert np.array_equal(got, O.inclusive_scan(x, op, init)), (dtype, op, n, cfg, "incl+init")
+                got = _scan_case(torch, q, scratch, x, op, "exclusive", init, config=cfg)
+                assert np.array_equal(got, O.exclusive_scan(x, init, op)), (dtype, op, n, cfg, "excl")
+    assert scratch.watchdog_trips() == 0
+
+
 @pytest.mark.parametrize("offset", [1, 2, 3])
 def test_scan_misaligned_and_in_place(gpu, q, scratch, offset):
     torch, _ = gpu
