@@ -256,9 +256,10 @@ __device__ __forceinline__ uint64_t clamp_off(const uint64_t *offsets, uint64_t 
 }
 
 // Setup: tile_seg[t] = the non-empty segment containing element t*tile_elems
-// (num_segments for t == num_tiles), the head bitmap (bit b for the first
-// element b of every non-empty segment; the bitmap is cleared beforehand),
-// and init into empty segments' results.
+// (num_segments for t == num_tiles), the head bitmap for the scans (bit b
+// for the first element b of every non-empty segment; the bitmap is cleared
+// beforehand; none for the reductions, heads_out NULL), and init into empty
+// segments' results.
 template <class T>
 __global__ void setup_kernel(params<T> p, uint32_t *tile_seg_out, uint32_t *heads_out, uint64_t tile_elems) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -270,7 +271,7 @@ __global__ void setup_kernel(params<T> p, uint32_t *tile_seg_out, uint32_t *head
       if (p.reduce) p.out[s] = static_cast<T>(p.init);
       continue;
     }
-    atomicOr(&heads_out[b >> 5], 1u << (b & 31));
+    if (heads_out) atomicOr(&heads_out[b >> 5], 1u << (b & 31));
     // tiles whose first element lies in [b, e)
     const uint64_t t0 = (b + tile_elems - 1) / tile_elems;
     const uint64_t t1 = (e + tile_elems - 1) / tile_elems;
@@ -677,14 +678,6 @@ __global__ __launch_bounds__(1024) void segmented_reduce_kernel(params<T> p,
   const uint32_t tile = blockIdx.x;
   if (tile >= p.num_tiles) return;
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
-  uint32_t hword0 = 0, hword1 = 0;
-  {
-    const uint64_t i0 = (base >> 5) + tid;
-    const uint64_t i1 = i0 + B;
-    if (static_cast<uint32_t>(tid) < head_words && i0 < p.head_words_total) hword0 = p.heads[i0];
-    if (static_cast<uint32_t>(tid + B) < head_words && i1 < p.head_words_total) hword1 = p.heads[i1];
-  }
-  __builtin_amdgcn_sched_barrier(0);
   uint64_t s_lo = p.tile_seg[tile];
   uint64_t s_hi = p.tile_seg[tile + 1];
 
@@ -704,15 +697,26 @@ __global__ __launch_bounds__(1024) void segmented_reduce_kernel(params<T> p,
         v[r][k] = (idx < p.total) ? elem_traits<T>::to_work(p.in[idx]) : id;
       }
   }
-  if (static_cast<uint32_t>(tid) < head_words) s_heads[tid] = hword0;
-  if (static_cast<uint32_t>(tid + B) < head_words) s_heads[tid + B] = hword1;
+  for (uint32_t i = tid; i < head_words; i += B) s_heads[i] = 0;
   if (s_lo >= p.num_segments) s_lo = p.num_segments - 1;
   if (s_hi >= p.num_segments) s_hi = p.num_segments - 1;
   if (s_hi < s_lo) s_hi = s_lo;
-  const bool off_in_lds = (s_hi - s_lo) < off_cap;
-  {
-    const uint64_t n_off = off_in_lds ? (s_hi - s_lo + 1) : 0;
-    for (uint64_t i = tid; i < n_off; i += B) s_off[i] = clamp_off(p.offsets, s_lo + i, p.total);
+  // Segments s_lo..s_hi touch the tile; their offsets and the next one's.
+  const uint64_t n_seg = s_hi - s_lo + 1;
+  const bool off_in_lds = n_seg + 1 <= off_cap;
+  if (off_in_lds)
+    for (uint64_t i = tid; i <= n_seg; i += B) s_off[i] = clamp_off(p.offsets, s_lo + i, p.total);
+  __syncthreads();
+  // The tile's head bits (tile elements 0..tile_elems), marked from the
+  // offsets: a reduction needs no global head bitmap, so its plan has no
+  // bitmap to clear and fill.
+  for (uint64_t i = tid; i < n_seg; i += B) {
+    const uint64_t b = off_in_lds ? s_off[i] : clamp_off(p.offsets, s_lo + i, p.total);
+    const uint64_t e = off_in_lds ? s_off[i + 1] : clamp_off(p.offsets, s_lo + i + 1, p.total);
+    if (e > b && b >= base && b - base <= tile_elems) {
+      const uint64_t pos = b - base;
+      atomicOr(&s_heads[pos >> 5], 1u << (pos & 31));
+    }
   }
   __syncthreads();
   if (full) {
@@ -885,8 +889,10 @@ acpp_status launch_rows(launch_plan<T, Op> &lp, hipStream_t stream, const char *
   return check_launch(name);
 }
 
-// Segmented reduction: clear the head bitmap, set up, one work-group per
-// tile, then the fix-up of segments that span tiles.
+// Segmented reduction: set up (tile -> first segment, init into empty
+// segments), one work-group per tile (its head bits marked in LDS from the
+// offsets, so no global bitmap is cleared or filled), then the fix-up of
+// segments that span tiles.
 template <class T, class Op>
 acpp_status enqueue_reduce(acpp_allocation_group *g, launch_plan<T, Op> &lp, hipStream_t stream, const T *in,
                            T *out, const uint64_t *offsets, uint64_t num_segments, uint64_t total,
@@ -902,8 +908,8 @@ acpp_status enqueue_reduce(acpp_allocation_group *g, launch_plan<T, Op> &lp, hip
   p.out = out;
   p.offsets = offsets;
   p.tile_seg = lp.tile_seg;
-  p.heads = lp.heads;
-  p.head_words_total = lp.head_words;
+  p.heads = nullptr;  // the reduce kernel marks its heads from the offsets
+  p.head_words_total = 0;
   p.block_lookback = 0;
   p.force_give_up = 0;
   p.num_segments = num_segments;
@@ -918,10 +924,8 @@ acpp_status enqueue_reduce(acpp_allocation_group *g, launch_plan<T, Op> &lp, hip
   p.status = nullptr;
   lp.grid = static_cast<uint32_t>(num_tiles);
 
-  st = fill_zero(lp.heads, lp.head_words * sizeof(uint32_t), stream);
-  if (st != ACPP_SUCCESS) return st;
   hipLaunchKernelGGL(setup_kernel<T>, dim3(static_cast<uint32_t>(std::min<uint64_t>(ceil_div(num_segments + 1, 256), 4096))),
-                     dim3(256), 0, stream, p, lp.tile_seg, lp.heads, lp.tile_elems);
+                     dim3(256), 0, stream, p, lp.tile_seg, static_cast<uint32_t *>(nullptr), lp.tile_elems);
   st = check_launch(name);
   if (st != ACPP_SUCCESS) return st;
   if (total > 0) {
@@ -971,13 +975,13 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   st = group_obtain(g, (num_tiles + 1) * sizeof(uint32_t), &ts);
   if (st != ACPP_SUCCESS) return st;
   lp.tile_seg = static_cast<uint32_t *>(ts);
-  // Head bitmap over elements [0, total] plus one spare word.
+  if (reduce) return enqueue_reduce<T, Op>(g, lp, stream, in, out, offsets, num_segments, total, init, num_tiles, name);
+  // Head bitmap over elements [0, total] plus one spare word (scans).
   lp.head_words = (total + 1 + 31) / 32 + 1;
   void *hb = nullptr;
   st = group_obtain(g, lp.head_words * sizeof(uint32_t), &hb);
   if (st != ACPP_SUCCESS) return st;
   lp.heads = static_cast<uint32_t *>(hb);
-  if (reduce) return enqueue_reduce<T, Op>(g, lp, stream, in, out, offsets, num_segments, total, init, num_tiles, name);
   using SIO = seg_status<typename elem_traits<T>::work>;
   static_assert(SIO::words <= seg_stride_words, "status slot too small");
   lp.region_used = sizeof(sync_header) + num_tiles * seg_stride_words * sizeof(uint64_t);
