@@ -245,10 +245,17 @@ acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
     if (s != ACPP_SUCCESS) return s;
     a.sync = true;
     a.state = std::make_shared<region_state>();
-    s = reset_sync_region(a.ptr, a.size, g->dev, stream);
-    if (s != ACPP_SUCCESS) {
-      g->parent->deallocate(a);
-      return s;
+    if (stream_is_capturing(stream)) {
+      // The captured launch clears the part it uses (begin_region_launch
+      // under capture), and the first eager launch clears all of it: a
+      // clear here would be a second one in every replay of the graph.
+      a.state->mode = region_state::graph;
+    } else {
+      s = reset_sync_region(a.ptr, a.size, g->dev, stream);
+      if (s != ACPP_SUCCESS) {
+        g->parent->deallocate(a);
+        return s;
+      }
     }
   }
   g->held.push_back(a);

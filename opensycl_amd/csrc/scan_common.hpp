@@ -190,24 +190,32 @@ __device__ W lookback(const uint64_t *status, uint32_t tile, uint32_t epoch,
 
 // Exclusive prefixes, in element order, of a tile's per-(row, wave) totals:
 // s_tot[r * nw + w] holds the totals on entry and the exclusive prefixes on
-// return (P = R * nw <= 128 entries).  Called by one whole wave, which scans
-// them with DPP (two entries per lane above 64); returns the tile aggregate,
-// uniform across the wave.  This replaces a walk over all R * nw totals by
-// every work-item (128 dependent LDS reads and combines per work-item for
-// 1024 x 8 tiles), which also delayed the aggregate's publication.
+// return (P = R * nw <= 256 entries).  Called by one whole wave, which scans
+// them with DPP (1, 2 or 4 consecutive entries per lane); returns the tile
+// aggregate, uniform across the wave.  This replaces a walk over all R * nw
+// totals by every work-item (128 dependent LDS reads and combines per
+// work-item for 1024 x 8 tiles), which also delayed the aggregate's
+// publication.
 template <class Op, class W>
 __device__ W tile_prefixes(int P, W *s_tot) {
   const int lane = wave::lane_id();
-  const int per = P > 64 ? 2 : 1;
+  const int per = P > 128 ? 4 : P > 64 ? 2 : 1;
   const int a = lane * per;
-  W v0 = Op::identity(), v1 = Op::identity();
-  if (a < P) v0 = s_tot[a];
-  if (per == 2 && a + 1 < P) v1 = s_tot[a + 1];
-  const W incl = wave::inclusive_scan<Op>(Op::apply(v0, v1));
-  const W excl = wave::shift_up<Op>(incl);
+  W v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = (j < per && a + j < P) ? s_tot[a + j] : Op::identity();
+  W sum = v[0];
+#pragma unroll
+  for (int j = 1; j < 4; ++j)
+    if (j < per) sum = Op::apply(sum, v[j]);
+  const W incl = wave::inclusive_scan<Op>(sum);
+  W excl = wave::shift_up<Op>(incl);
   const W agg = wave::read_lane(incl, 63);
-  if (a < P) s_tot[a] = excl;
-  if (per == 2 && a + 1 < P) s_tot[a + 1] = Op::apply(excl, v0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j < per && a + j < P) s_tot[a + j] = excl;
+    excl = Op::apply(excl, v[j]);
+  }
   return agg;
 }
 

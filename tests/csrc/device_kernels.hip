@@ -5,11 +5,13 @@
 //   tests/sycl/group_functions/group_functions_{reduce,scan}.cpp
 //   tests/compiler/cbs/{reduce_for,babelstream_dot}.cpp
 //   tests/sycl/atomic.cpp, tests/pstl/std_atomic.cpp
+//   tests/sycl/reduction.cpp (reducers inside a kernel)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "acpp/device/atomic_ref.hpp"
 #include "acpp/device/group_functions.hpp"
+#include "acpp/device/reducer.hpp"
 
 namespace {
 
@@ -415,6 +417,69 @@ int launch_movement(int level, int which, uint32_t param, const void *in, const 
 
 }  // namespace
 
+// A user kernel with a reduction: grid-stride per-work-item reducers, then
+// combine_into.  The reducer's named operator (+=, *=, &=, |=, ^=) is used
+// where it has one; `user_identity` starts the reducers at an explicit
+// identity instead of the known one.
+template <class T, class Op>
+__device__ void reducer_step(acpp::reducer<T, Op> &r, const T &v) {
+  if constexpr (std::is_same_v<Op, acpp::plus<T>>)
+    r += v;
+  else if constexpr (std::is_same_v<Op, acpp::multiplies<T>>)
+    r *= v;
+  else if constexpr (std::is_integral_v<T> && std::is_same_v<Op, acpp::bit_and<T>>)
+    r &= v;
+  else if constexpr (std::is_integral_v<T> && std::is_same_v<Op, acpp::bit_or<T>>)
+    r |= v;
+  else if constexpr (std::is_integral_v<T> && std::is_same_v<Op, acpp::bit_xor<T>>)
+    r ^= v;
+  else
+    r.combine(v);
+}
+
+template <class T, class Op>
+__global__ void reducer_kernel(const T *x, uint64_t n, T *out, int user_identity) {
+  acpp::reducer<T, Op> r = user_identity ? acpp::reducer<T, Op>(acpp::known_identity<Op, T>::get())
+                                         : acpp::reducer<T, Op>();
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    reducer_step(r, x[i]);
+  acpp::combine_into(acpp::this_group(), r, out);
+}
+
+template <class T>
+int launch_reducer(int op, const void *x, uint64_t n, void *out, uint32_t groups, uint32_t local,
+                   int user_identity, hipStream_t s) {
+  if (local == 0 || local > 1024 || local % 64 != 0 || groups == 0) return 1;
+  const T *xp = static_cast<const T *>(x);
+  T *op_ = static_cast<T *>(out);
+#define ACPP_TK_REDUCER(OP)                                                                        \
+  hipLaunchKernelGGL((reducer_kernel<T, OP<T>>), dim3(groups), dim3(local), 0, s, xp, n, op_, user_identity); \
+  break;
+  switch (op) {
+    case 0: ACPP_TK_REDUCER(acpp::plus)
+    case 1: ACPP_TK_REDUCER(acpp::multiplies)
+    case 2: ACPP_TK_REDUCER(acpp::minimum)
+    case 3: ACPP_TK_REDUCER(acpp::maximum)
+    case 7: ACPP_TK_REDUCER(acpp::logical_and)
+    case 8: ACPP_TK_REDUCER(acpp::logical_or)
+    default:
+      if constexpr (std::is_integral_v<T>) {
+        switch (op) {
+          case 4: ACPP_TK_REDUCER(acpp::bit_and)
+          case 5: ACPP_TK_REDUCER(acpp::bit_or)
+          case 6: ACPP_TK_REDUCER(acpp::bit_xor)
+          default: return 1;
+        }
+        break;
+      } else {
+        return 1;
+      }
+  }
+#undef ACPP_TK_REDUCER
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
 extern "C" {
 
 // tag: 0 i8, 1 i32, 2 i64, 3 u32, 4 f32, 5 f64, 6 f16
@@ -498,6 +563,21 @@ int tk_atomic_single(int tag, int op, void *data, const void *operand, void *pre
     default: return 1;
   }
   return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+// acpp::reducer + combine_into.  tag: 1 i32, 2 i64, 3 u32, 4 f32, 5 f64, 6 u64.
+int tk_reducer(int tag, int op, const void *x, uint64_t n, void *out, uint32_t groups, uint32_t local,
+               int user_identity, void *stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (tag) {
+    case 1: return launch_reducer<int32_t>(op, x, n, out, groups, local, user_identity, s);
+    case 2: return launch_reducer<int64_t>(op, x, n, out, groups, local, user_identity, s);
+    case 3: return launch_reducer<uint32_t>(op, x, n, out, groups, local, user_identity, s);
+    case 4: return launch_reducer<float>(op, x, n, out, groups, local, user_identity, s);
+    case 5: return launch_reducer<double>(op, x, n, out, groups, local, user_identity, s);
+    case 6: return launch_reducer<uint64_t>(op, x, n, out, groups, local, user_identity, s);
+    default: return 1;
+  }
 }
 
 // 8/16-bit builtins.  tag: 7 i8, 8 i16, 9 u8, 10 u16.

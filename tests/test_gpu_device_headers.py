@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from gpu_util import dev, host
+from gpu_util import ALL_OPS, dev, host, op_input
 
 pytestmark = pytest.mark.gpu
 
@@ -48,6 +48,8 @@ def tk():
     lib.tk_atomic_fold_narrow.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_uint64,
                                           ctypes.c_uint32, vp]
     lib.tk_atomic_single_narrow.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    lib.tk_reducer.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_uint64, vp, ctypes.c_uint32,
+                               ctypes.c_uint32, ctypes.c_int, vp]
     return lib
 
 
@@ -367,3 +369,39 @@ def test_narrow_atomic_builtins_single(gpu, tk, dtype):
         want = init.copy()
         want[k] = -7
         assert np.array_equal(host(d), want), (dtype, k)
+
+
+# acpp::reducer inside a user kernel (sycl::reduction's device side,
+# reduction.hpp:61-160): every work-item folds its grid-stride share, the
+# group's total reaches *out through combine_into.  *out starts at a
+# non-identity value, so the result is op(old, fold of x) as for a
+# reduction without initialize_to_identity.
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.uint32, np.uint64, np.float32, np.float64])
+def test_reducer_in_user_kernel(gpu, tk, dtype):
+    torch, _ = gpu
+    ops = ALL_OPS if np.issubdtype(dtype, np.integer) else [0, 1, 2, 3, 7, 8]
+    for op in ops:
+        for n, groups, local in ((1, 1, 64), (1000, 3, 256), (100003, 40, 512), (1 << 20, 256, 1024)):
+            x = op_input(dtype, op, n, seed=n + 17 * op)
+            if np.issubdtype(dtype, np.floating) and op == 0:
+                x = np.round(x * 64).astype(dtype)  # small integers: sums are exact
+            old = dtype(3) if op not in (7, 8) else dtype(1)
+            if np.issubdtype(dtype, np.floating) and op in (7, 8):
+                # logical operators on floats give 0 or 1 (T(a && b)); the
+                # oracle's reduce takes them for integers only
+                nz = x != 0
+                want = dtype(bool(old) and bool(nz.all())) if op == 7 else dtype(bool(old) or bool(nz.any()))
+            else:
+                want = O.reduce(x, old, op)
+            for user_identity in (0, 1):
+                out = dev(torch, np.array([old], dtype=dtype))
+                xd = dev(torch, x)
+                rc = tk.tk_reducer(ATOMIC_TAGS[dtype], op, xd.data_ptr(), n, out.data_ptr(), groups, local,
+                                   user_identity, _stream(torch))
+                assert rc == 0
+                torch.cuda.synchronize()
+                got = host(out)[0]
+                if np.issubdtype(dtype, np.floating) and op == 1:
+                    assert np.isclose(got, want, rtol=1e-3), (dtype, op, n, got, want)
+                else:
+                    assert got == want, (dtype, op, n, user_identity, got, want)
