@@ -122,7 +122,7 @@ __device__ void atomic_combine(T *out, T v) {
 template <class T, class BinaryOperation>
 __device__ void combine_into(group g, const reducer<T, BinaryOperation> &r, T *out) {
   const T total = reduce_over_group(g, r.value(), BinaryOperation{});
-  if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) detail::atomic_combine<T, BinaryOperation>(out, total);
+  if (g.leader()) detail::atomic_combine<T, BinaryOperation>(out, total);
 }
 
 }  // namespace acpp

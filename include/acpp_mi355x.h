@@ -367,7 +367,9 @@ acpp_status acpp_exclusive_scan_i32(acpp_allocation_group *scratch, acpp_stream_
    tiles); larger inputs are rejected with ACPP_ERROR_INVALID_ARGUMENT.
    Calls on one GPU share that array: calls whose kernels overlap on one GPU
    (different streams) race and give wrong results; calls ordered on a
-   stream, or otherwise serialised, are safe.  Calls no hipMalloc/hipFree. */
+   stream, or otherwise serialised, are safe, graph replays of a captured
+   call included (after a capture, eager calls clear the array first).
+   Calls no hipMalloc/hipFree. */
 acpp_status acpp_exclusive_scan_i32_noalloc(acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, int32_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, int64_t init, acpp_algorithm_op op, const acpp_launch_config *config);
 acpp_status acpp_exclusive_scan_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *d_first, uint8_t init, acpp_algorithm_op op, const acpp_launch_config *config);

@@ -185,6 +185,12 @@ class AllocationGroup:
         return p.value
 
     def release(self) -> None:
+        """Return the group's regions to the cache now.  Refused while a
+        SegmentedPlan captured from this group is alive: its graph still
+        uses them, and replays would corrupt whichever call got them next."""
+        if self._plans > 0:
+            raise RuntimeError("AllocationGroup.release: a SegmentedPlan captured from this group is alive; "
+                               "close the plan first")
         check(_abi.load().acpp_allocation_group_release(self._h))
 
     def watchdog_trips(self) -> int:

@@ -149,6 +149,12 @@ struct region_state {
   int mode = fresh;
   uint32_t epoch = 0;        // epoch of the last launch (host_managed)
   uint32_t ticket_base = 0;  // tile ticket counter value before the next launch
+  // A captured launch uses the region: every replay of that graph clears
+  // the region and draws tickets from 0, unseen by the host, so eager
+  // launches clear it first as long as the graph may run.  Set at capture;
+  // cleared when the region is handed out by the cache again (a graph must
+  // not outlive the group that holds its regions).
+  bool captured = false;
 };
 
 // Obtain ordinary scratch from the group.

@@ -240,7 +240,14 @@ acpp_status group_obtain_sync(acpp_allocation_group *g, size_t bytes,
                      "ACPP_ALLOCATION_DEVICE or _SHARED cache");
   allocation a;
   bytes = round_request(bytes);
-  if (!g->parent->find(bytes, 256, g->dev, true, a)) {
+  if (g->parent->find(bytes, 256, g->dev, true, a)) {
+    if (a.state && a.state->captured) {
+      // Back from a group that held it for a graph: that graph is gone, so
+      // one clear before the next eager launch suffices.
+      a.state->captured = false;
+      a.state->mode = region_state::graph;
+    }
+  } else {
     acpp_status s = g->parent->allocate(bytes, g->dev, a);
     if (s != ACPP_SUCCESS) return s;
     a.sync = true;
@@ -278,8 +285,9 @@ acpp_status begin_region_launch_flag(uint32_t *trip_flag, void *region, size_t u
     acpp_status st = reset_sync_region_flag(region, used, trip_flag, stream);
     if (st != ACPP_SUCCESS) return st;
     rs->mode = region_state::graph;
+    rs->captured = true;
   } else {
-    if (rs->mode == region_state::graph || rs->epoch >= max_epoch) {
+    if (rs->mode == region_state::graph || rs->captured || rs->epoch >= max_epoch) {
       acpp_status st = reset_sync_region_flag(region, capacity, trip_flag, stream);
       if (st != ACPP_SUCCESS) return st;
       rs->epoch = 0;

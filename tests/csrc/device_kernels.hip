@@ -211,6 +211,53 @@ __global__ void atomic_fold_kernel(int op, T *data, uint64_t n) {
   }
 }
 
+// Work-group-local atomics (tests/sycl/atomic.cpp's local-memory cases):
+// every work-group folds its slice data[g*L+1 .. g*L+L-1] into an LDS value
+// that starts at data[g*L], through atomic_ref<T, relaxed, work_group,
+// local_space>, and work-item 0 writes the value to out[g].
+template <class T>
+__global__ void atomic_fold_local_kernel(int op, const T *data, T *out) {
+  __shared__ T acc;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;
+  if (threadIdx.x == 0) acc = data[base];
+  __syncthreads();
+  if (threadIdx.x != 0) {
+    const T x = data[base + threadIdx.x];
+    acpp::atomic_ref<T, acpp::memory_order::relaxed, acpp::memory_scope::work_group,
+                     acpp::address_space::local_space>
+        r{acc};
+    if constexpr (std::is_integral_v<T>) {
+      switch (op) {
+        case 0: r.fetch_and(x); break;
+        case 1: r.fetch_or(x); break;
+        case 2: r.fetch_xor(x); break;
+        default: break;
+      }
+    }
+    switch (op) {
+      case 3: r.fetch_add(x); break;
+      case 4: r.fetch_sub(x); break;
+      case 5: r.fetch_min(x); break;
+      case 6: r.fetch_max(x); break;
+      default: break;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+
+// atomic_ref on a pointer: every work-item advances *slot by one element
+// (fetch_add(1) on an int*, so the address moves by sizeof(int)), and
+// records the value it saw in seen[i].
+__global__ void atomic_pointer_kernel(int **slot, int **seen, uint64_t n) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  acpp::atomic_ref<int *, acpp::memory_order::relaxed, acpp::memory_scope::device,
+                   acpp::address_space::global_space>
+      r{*slot};
+  seen[i] = r.fetch_add(1);
+}
+
 // 8- and 16-bit atomic builtins: work-items slots..n-1 apply the op with
 // operand data[i] to data[i % slots].  With slots < 4 bytes' worth of
 // elements, neighbouring elements of one 32-bit word are updated
@@ -562,6 +609,36 @@ int tk_atomic_single(int tag, int op, void *data, const void *operand, void *pre
     case 6: hipLaunchKernelGGL(atomic_single_kernel<unsigned long long>, dim3(1), dim3(1), 0, s, op, static_cast<unsigned long long *>(data), *static_cast<const unsigned long long *>(operand), static_cast<unsigned long long *>(prev)); break;
     default: return 1;
   }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+// Local-memory atomic_ref folds.  tag: 1 i32, 2 i64, 3 u32, 4 f32, 5 f64, 6 u64;
+// groups x local elements, one result per group.
+int tk_atomic_fold_local(int tag, int op, const void *data, void *out, uint32_t groups, uint32_t local,
+                         void *stream) {
+  if (local == 0 || local > 1024 || groups == 0) return 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define ACPP_TK_LOCAL(T)                                                                                   \
+  hipLaunchKernelGGL(atomic_fold_local_kernel<T>, dim3(groups), dim3(local), 0, s, op,                     \
+                     static_cast<const T *>(data), static_cast<T *>(out));                                 \
+  break;
+  switch (tag) {
+    case 1: ACPP_TK_LOCAL(int32_t)
+    case 2: ACPP_TK_LOCAL(int64_t)
+    case 3: ACPP_TK_LOCAL(uint32_t)
+    case 4: ACPP_TK_LOCAL(float)
+    case 5: ACPP_TK_LOCAL(double)
+    case 6: ACPP_TK_LOCAL(uint64_t)
+    default: return 1;
+  }
+#undef ACPP_TK_LOCAL
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int tk_atomic_pointer(void *slot, void *seen, uint64_t n, void *stream) {
+  const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
+  hipLaunchKernelGGL(atomic_pointer_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<int **>(slot), static_cast<int **>(seen), n);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

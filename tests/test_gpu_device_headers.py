@@ -50,6 +50,9 @@ def tk():
     lib.tk_atomic_single_narrow.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp]
     lib.tk_reducer.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_uint64, vp, ctypes.c_uint32,
                                ctypes.c_uint32, ctypes.c_int, vp]
+    lib.tk_atomic_fold_local.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_uint32,
+                                         ctypes.c_uint32, vp]
+    lib.tk_atomic_pointer.argtypes = [vp, vp, ctypes.c_uint64, vp]
     return lib
 
 
@@ -369,6 +372,46 @@ def test_narrow_atomic_builtins_single(gpu, tk, dtype):
         want = init.copy()
         want[k] = -7
         assert np.array_equal(host(d), want), (dtype, k)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.uint32, np.uint64, np.float32,
+                                   np.float64])
+def test_atomic_ref_local_space_fold(gpu, tk, dtype):
+    # atomic_ref<T, relaxed, work_group, local_space> on an LDS value
+    # (atomic_ref.hpp:58-61 allows the local space): each work-group folds
+    # its own slice, starting from the slice's first element.
+    torch, _ = gpu
+    groups, local = 37, 256
+    vals = (-np.arange(groups * local, dtype=np.int64) % 1000 - 500).astype(dtype)
+    names = ["add", "sub", "min", "max"] + ([] if np.issubdtype(dtype, np.floating)
+                                            else ["and", "or", "xor"])
+    for name in names:
+        d = dev(torch, vals)
+        out = torch.empty(groups, dtype=d.dtype, device="cuda")
+        assert tk.tk_atomic_fold_local(ATOMIC_TAGS[dtype], RMW[name], d.data_ptr(), out.data_ptr(), groups,
+                                       local, _stream(torch)) == 0
+        torch.cuda.synchronize()
+        got = host(out)
+        rmw = {"and": O.RMW_AND, "or": O.RMW_OR, "xor": O.RMW_XOR, "add": O.RMW_ADD, "sub": O.RMW_SUB,
+               "min": O.RMW_MIN, "max": O.RMW_MAX}[name]
+        for g in range(groups):
+            sl = vals[g * local:(g + 1) * local]
+            assert got[g] == O.atomic_fold(rmw, sl[1:], sl[0]), (dtype, name, g)
+
+
+def test_atomic_ref_pointer(gpu, tk):
+    # atomic_ref<int*>: fetch_add(1) moves the pointer by sizeof(int); every
+    # work-item sees a distinct old value and the final one is base + n.
+    torch, _ = gpu
+    n = 100003
+    base = torch.zeros(n + 1, dtype=torch.int32, device="cuda")
+    slot = torch.tensor([base.data_ptr()], dtype=torch.int64, device="cuda")
+    seen = torch.zeros(n, dtype=torch.int64, device="cuda")
+    assert tk.tk_atomic_pointer(slot.data_ptr(), seen.data_ptr(), n, _stream(torch)) == 0
+    torch.cuda.synchronize()
+    assert int(slot.item()) == base.data_ptr() + 4 * n
+    offs = (host(seen) - base.data_ptr()) // 4
+    assert np.array_equal(np.sort(offs), np.arange(n))
 
 
 # acpp::reducer inside a user kernel (sycl::reduction's device side,

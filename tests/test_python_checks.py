@@ -83,3 +83,16 @@ def test_group_release_waits_for_plans():
     p.close()
     assert g._plans == 0
     np.testing.assert_equal(p._scratch, None)
+
+
+def test_release_refused_while_a_plan_is_alive():
+    # a captured graph keeps using the group's regions: releasing them to the
+    # cache before the plan is closed would hand them to other calls
+    g = acpp.AllocationGroup.__new__(acpp.AllocationGroup)
+    g._h = None
+    g._plans = 1
+    g._close_pending = False
+    with pytest.raises(RuntimeError, match="SegmentedPlan"):
+        g.release()
+    g.close()  # deferred while the plan lives
+    assert g._close_pending

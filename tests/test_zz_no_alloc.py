@@ -58,3 +58,42 @@ def test_noalloc_scan_size_limit(gpu, q, scratch):
     assert st == _abi.ERROR_INVALID_ARGUMENT
     msg = lib.acpp_last_error_message().decode()
     assert "16384 tiles" in msg, msg
+
+
+def test_noalloc_scan_under_graph_capture(gpu, q, scratch):
+    # The callers this entry point is for: a scan captured into a hipGraph
+    # (through torch's graph capture) and replayed.  Under capture the
+    # library clears its static scratch with a kernel node; every replay,
+    # with new input written between replays and eager work in between, must
+    # match the allocation-group scan of the same input.
+    torch, _ = gpu
+    lib = _abi.load()
+    n = (1 << 22) + 333
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    x = torch.randint(-(1 << 15), (1 << 15) + 1, (n,), dtype=torch.int32, device="cuda", generator=g)
+    y = torch.empty_like(x)
+    _abi.check(_noalloc(lib, torch, x, y, 1))  # eager first call: per-device set-up
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            st = _noalloc(lib, torch, x, y, 1)
+    assert st == 0, lib.acpp_last_error_message().decode()
+    torch.cuda.current_stream().wait_stream(side)
+    for rep in range(4):
+        x.copy_(torch.randint(-(1 << 15), (1 << 15) + 1, (n,), dtype=torch.int32, device="cuda", generator=g))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        want = torch.empty_like(x)
+        acpp.exclusive_scan(q, scratch, x, want, 1)
+        q.wait()
+        assert torch.equal(y, want), rep
+        # eager use of the same static scratch between replays
+        z = torch.empty_like(x)
+        _abi.check(_noalloc(lib, torch, x, z, 1))
+        torch.cuda.synchronize()
+        assert torch.equal(z, want), rep
