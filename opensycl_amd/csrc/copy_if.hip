@@ -13,10 +13,11 @@
 // counts with popcount, the work-group ranks the selected elements with DPP
 // wave scans and one LDS exchange, one wave looks back over the tile
 // counts (64-bit, one 128-byte status line per tile) for the tile's output
-// offset.  The tile's selected elements are compacted in LDS (128 KiB at
-// the default 1024 x 8 tile) and stored with consecutive work-items on
-// consecutive output elements (coalesced whatever the selection pattern).
-// The last tile writes the total count.
+// offset.  Meanwhile every wave compacts the selected elements of each of
+// its rows into that row's own LDS segment (128 KiB of segments at the
+// default 1024 x 8 tile); the segments are then stored with consecutive
+// lanes on consecutive output elements (coalesced whatever the selection
+// pattern).  The last tile writes the total count.
 //
 // The predicate is data, not a functor (a functor cannot cross the C-ABI):
 // comparison with a constant, a bit-mask test, or a stencil array for any
@@ -39,7 +40,6 @@ struct params {
   T *out;
   uint64_t *num_copied;     // may be NULL
   uint64_t n;
-  int pred;
   T operand;
   T mask;
   int vec_io;
@@ -102,37 +102,35 @@ __device__ __forceinline__ void select_bits(const params<T> &p, const T (&v)[R][
   }
 }
 
-template <class T, int R>
-__global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
+// Shared words of a copy_if work-group.
+template <int R>
+struct tile_shared {
+  uint32_t tile;
+  uint32_t wave[R * scan_detail::max_waves];  // per (row, wave): counts, then offsets
+  uint32_t agg;
+  uint64_t excl;
+};
+
+// One tile.  PRED and FULL are compile-time: a full tile has no bounds
+// checks and one predicate's code, since with one work-group per CU (the
+// LDS segments take 128 KiB) this phase is not overlapped with memory and
+// its instruction count is on the critical path.
+template <int PRED, bool FULL, class T, int R>
+__device__ __forceinline__ void copy_if_tile(const params<T> &p, uint32_t tile, tile_shared<R> &sh, T *s_stage) {
   using CountOp = ops::plus<uint32_t, uint32_t>;
   using SIO = scan_detail::status_io<uint64_t>;
   constexpr int VEC = 16 / sizeof(T);
+  constexpr int SEG = 64 * VEC;  // LDS room of one wave row's selection
   const int tid = threadIdx.x;
   const int B = blockDim.x;
   const int nw = B >> 6;
   const int w = tid >> 6;
   const int lane = tid & 63;
-
-  __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_wave[R * scan_detail::max_waves];  // per (row, wave): counts, then offsets
-  __shared__ uint32_t s_agg;
-  __shared__ uint64_t s_excl;
-  extern __shared__ unsigned char s_stage_raw[];  // the compacted tile: B * R * VEC elements
-  T *s_stage = reinterpret_cast<T *>(s_stage_raw);
-
-  if (tid == 0)
-    s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
-             p.ticket_base;
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  if (tile >= p.num_tiles) return;
-
   const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
-  const bool full = p.vec_io && (base + tile_elems <= p.n);
 
   T v[R][VEC];
-  if (full) {
+  if constexpr (FULL) {
     const scan_detail::u32x4_a4 *src = reinterpret_cast<const scan_detail::u32x4_a4 *>(p.in + base);
     u32x4 q[R];
 #pragma unroll
@@ -148,81 +146,98 @@ __global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
         v[r][k] = (idx < p.n) ? p.in[idx] : T(0);
       }
   }
-  // Selection bits; one instance of the loop per predicate kind (the
-  // switch is outside the per-element work).
   uint32_t fm[R];
-  switch (p.pred) {
-    case ACPP_PRED_NONZERO: select_bits<ACPP_PRED_NONZERO>(p, v, base, full, fm); break;
-    case ACPP_PRED_EQ: select_bits<ACPP_PRED_EQ>(p, v, base, full, fm); break;
-    case ACPP_PRED_NE: select_bits<ACPP_PRED_NE>(p, v, base, full, fm); break;
-    case ACPP_PRED_LT: select_bits<ACPP_PRED_LT>(p, v, base, full, fm); break;
-    case ACPP_PRED_LE: select_bits<ACPP_PRED_LE>(p, v, base, full, fm); break;
-    case ACPP_PRED_GT: select_bits<ACPP_PRED_GT>(p, v, base, full, fm); break;
-    case ACPP_PRED_GE: select_bits<ACPP_PRED_GE>(p, v, base, full, fm); break;
-    case ACPP_PRED_MASK_EQ: select_bits<ACPP_PRED_MASK_EQ>(p, v, base, full, fm); break;
-    default: select_bits<ACPP_PRED_STENCIL>(p, v, base, full, fm); break;
-  }
-  // Ranks inside the tile.
+  select_bits<PRED>(p, v, base, FULL, fm);
+  // Ranks inside each wave row; the (row, wave) counts go to LDS.
   uint32_t wexcl[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint32_t incl = wave::inclusive_scan<CountOp>(static_cast<uint32_t>(__builtin_popcount(fm[r])));
-    wexcl[r] = wave::shift_up<CountOp>(incl);
-    if (lane == 63) s_wave[r * nw + w] = incl;
+    const uint32_t c = static_cast<uint32_t>(__builtin_popcount(fm[r]));
+    const uint32_t incl = wave::inclusive_scan<CountOp>(c);
+    wexcl[r] = incl - c;
+    if (lane == 63) sh.wave[r * nw + w] = incl;
   }
   __syncthreads();
   // Wave 0 turns the (row, wave) counts into offsets and publishes the
-  // tile's count at once.
+  // tile's count before anything else, since every later tile's look-back
+  // may be waiting for it.
+  uint32_t agg0 = 0;
   if (w == 0) {
-    const uint32_t agg0 = scan_detail::tile_prefixes<CountOp, uint32_t>(R * nw, s_wave);
+    agg0 = scan_detail::tile_prefixes<CountOp, uint32_t>(R * nw, sh.wave);
     if (lane == 0) {
       if (tile == 0) {
         if (p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, static_cast<uint64_t>(agg0));
-        s_excl = 0;
+        sh.excl = 0;
       } else {
         SIO::publish(p.status, tile, p.epoch, st_aggregate, static_cast<uint64_t>(agg0));
       }
-      s_agg = agg0;
+      sh.agg = agg0;
     }
   }
-  __syncthreads();
-  uint32_t before[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) before[r] = s_wave[r * nw + w];
-  const uint64_t agg = s_agg;
-  // Wave 0 looks back for the tile's output offset; the compaction of the
-  // tile's selected elements into LDS needs only positions inside the tile,
-  // so every wave does its share meanwhile (wave 0 after its look-back).
-  if (w == 0 && tile != 0) {
-    wait_own_stores();  // the tile's count is out before the first poll
-    const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t, scan_detail::lookback_slots>(
-        p.status, tile, p.epoch, p.sync, p.force_give_up != 0);
-    if (lane == 0) {
-      if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, p.epoch, st_prefix, e + agg);
-      s_excl = e;
-    }
-  }
+  // Compaction: each wave row's selected elements go, in order, to that
+  // row's own LDS segment (room for all 64 x VEC of them), at the rank
+  // inside the wave row.  It needs no tile-wide offsets, so it runs while
+  // wave 0 looks back; the stores below then copy segment after segment.
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    uint32_t pos = before[r] + wexcl[r];
+    uint32_t pos = static_cast<uint32_t>((r * nw + w) * SEG) + wexcl[r];
     const uint32_t bits = fm[r];
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
       if (bits & (1u << k)) s_stage[pos++] = v[r][k];
     }
   }
+  // Wave 0 looks back for the tile's output offset.
+  if (w == 0 && tile != 0) {
+    wait_own_stores();  // the tile's count is out before the first poll
+    const uint64_t e = scan_detail::lookback<ops::plus<uint64_t, uint64_t>, uint64_t, scan_detail::lookback_slots>(
+        p.status, tile, p.epoch, p.sync, p.force_give_up != 0);
+    if (lane == 0) {
+      if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, p.epoch, st_prefix, e + agg0);
+      sh.excl = e;
+    }
+  }
   __syncthreads();
-  // Store the tile's output range from LDS with consecutive work-items on
-  // consecutive elements: coalesced whatever the selection pattern, and
-  // conflict-free in LDS.  Rows of 16-byte stores read the compacted tile
-  // with a stride of four words between lanes instead; half of copy_if's
-  // LDS cycles were then bank conflicts, and the 2^28 int32 compaction took
-  // 369 us against 357 us this way.
-  const uint64_t tile_prefix = s_excl;
-  if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + agg;
+  // Each wave stores its own segments: a segment's elements are contiguous
+  // in the output, at the offset of its (row, wave) pair, and consecutive
+  // lanes take consecutive elements (coalesced stores, conflict-free LDS
+  // reads).
+  const uint32_t count = __builtin_amdgcn_readfirstlane(sh.agg);
+  const uint64_t ex = sh.excl;
+  const uint64_t tile_prefix =
+      (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ex >> 32))) << 32) |
+      __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ex));
+  if (tile + 1 == p.num_tiles && tid == 0 && p.num_copied) *p.num_copied = tile_prefix + count;
   T *dst = p.out + tile_prefix;
-  const uint32_t count = static_cast<uint32_t>(agg);
-  for (uint32_t i = tid; i < count; i += B) dst[i] = s_stage[i];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int seg = r * nw + w;
+    const uint32_t off = sh.wave[seg];
+    const uint32_t end = (seg + 1 < R * nw) ? sh.wave[seg + 1] : count;
+    const T *src = s_stage + seg * SEG;
+    for (uint32_t j = lane; j < end - off; j += 64) dst[off + j] = src[j];
+  }
+}
+
+template <class T, int R, int PRED>
+__global__ __launch_bounds__(1024) void copy_if_kernel(params<T> p) {
+  constexpr int VEC = 16 / sizeof(T);
+  __shared__ tile_shared<R> sh;
+  extern __shared__ unsigned char s_stage_raw[];  // R x nw segments of 64 x VEC elements (B * R * VEC)
+  T *s_stage = reinterpret_cast<T *>(s_stage_raw);
+
+  if (threadIdx.x == 0)
+    sh.tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+              p.ticket_base;
+  __syncthreads();
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(sh.tile);
+  if (tile >= p.num_tiles) return;
+  const uint64_t tile_elems = static_cast<uint64_t>(blockDim.x) * R * VEC;
+  const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+  if (p.vec_io && base + tile_elems <= p.n)
+    copy_if_tile<PRED, true>(p, tile, sh, s_stage);
+  else
+    copy_if_tile<PRED, false>(p, tile, sh, s_stage);
 }
 
 __global__ void store_zero_kernel(uint64_t *p) { *p = 0; }
@@ -281,7 +296,6 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   p.out = out;
   p.num_copied = num_copied;
   p.n = n;
-  p.pred = pred;
   p.stencil_vec = stencil && (reinterpret_cast<uintptr_t>(stencil) % VEC) == 0;
   p.operand = operand;
   p.mask = mask;
@@ -295,8 +309,39 @@ acpp_status entry(acpp_allocation_group *g, hipStream_t stream, const T *in, uin
   p.sync = static_cast<sync_header *>(region);
   p.status = reinterpret_cast<uint64_t *>(static_cast<char *>(region) + sizeof(sync_header));
   const size_t stage_bytes = static_cast<size_t>(tile_elems) * sizeof(T);  // 128 KiB at 1024 x 8
-  hipLaunchKernelGGL((copy_if_kernel<T, rows<T>()>), dim3(static_cast<uint32_t>(num_tiles)), dim3(wg),
-                     stage_bytes, stream, p);
+  const dim3 grid(static_cast<uint32_t>(num_tiles));
+  constexpr int R = rows<T>();
+  switch (pred) {
+    case ACPP_PRED_NONZERO:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_NONZERO>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    case ACPP_PRED_EQ:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_EQ>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    case ACPP_PRED_NE:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_NE>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    case ACPP_PRED_LT:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_LT>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    case ACPP_PRED_LE:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_LE>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    case ACPP_PRED_GT:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_GT>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    case ACPP_PRED_GE:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_GE>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    case ACPP_PRED_MASK_EQ:
+      // rejected above for floating-point elements
+      if constexpr (std::is_integral_v<T>)
+        hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_MASK_EQ>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+    default:
+      hipLaunchKernelGGL((copy_if_kernel<T, R, ACPP_PRED_STENCIL>), grid, dim3(wg), stage_bytes, stream, p);
+      break;
+  }
   st = check_launch(name);
   end_region_launch(rs, rl, static_cast<uint32_t>(num_tiles), st);
   return st == ACPP_SUCCESS ? clear_error() : st;
