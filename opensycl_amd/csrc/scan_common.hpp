@@ -325,54 +325,49 @@ struct params {
   uint64_t *status;
 };
 
+// Shared words of a scan work-group.
+template <class W, int R>
+struct scan_shared {
+  uint32_t tile;
+  W wave[R * max_waves];  // per (row, wave): totals, then exclusive offsets
+  W excl;
+  W agg;
+  uint64_t masks[2 * max_waves];
+  W red[max_waves];
+};
+
+// One tile.  FULL tiles (all R x B vectors in range, 16-byte I/O allowed)
+// and the partial last tile are separate compile-time paths: with no join
+// after the loads, the compiler waits for each row's load only when that
+// row is processed (vmcnt(7), (6), ...), so the local and wave scans of
+// the first rows overlap the arrival of the later ones.  With one join it
+// waited for all R loads first (2^28 int32: 368-377 us split, 373-379 us
+// joined, same runs; 2^20: 11.4 against 12.0 us).
 // XF: apply the element transform p.unary to every input element (the
 // transform scans); false for the plain scans.
 // NT: full tiles are read and written with non-temporal loads and stores.
-template <class T, class Op, int R, bool XF = false, bool NT = false>
-__global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
+template <bool FULL, class T, class Op, int R, bool XF, bool NT>
+__device__ __forceinline__ void scan_tile(const params<T> &p, uint32_t tile,
+                                          scan_shared<typename elem_traits<T>::work, R> &sh) {
   using W = typename elem_traits<T>::work;
   using SIO = status_io<W>;
   constexpr int VEC = 16 / sizeof(T);
-
   const int tid = threadIdx.x;
   const int B = blockDim.x;
   const int nw = B >> 6;
   const int w = tid >> 6;
   const int lane = tid & 63;
-
-  __shared__ uint32_t s_tile;
-  __shared__ W s_wave[R * max_waves];  // per (row, wave): totals, then exclusive offsets
-  __shared__ W s_excl;
-  __shared__ W s_agg;
-  __shared__ uint64_t s_masks[2 * max_waves];
-  __shared__ W s_red[max_waves];
-
-  // Dynamic tile ids: tiles are handed out in the order work-groups start,
-  // so every predecessor of a tile is already running.  The counter is never
-  // reset; the host passes the value it had before this launch.
-  if (tid == 0) {
-    s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT) -
-             p.ticket_base;
-  }
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  // Tickets beyond this launch's tiles only appear if the host-side ticket
-  // count went out of step with the counter; never touch memory for them.
-  if (tile >= p.num_tiles) return;
   const uint32_t epoch = p.epoch;
-
   const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
-  const bool full = p.vec_io && (base + tile_elems <= p.n);
+  // Elements of the tile in range (partial tiles; < tile_elems there), so
+  // that bounds checks compare 32-bit offsets inside the tile.
+  const uint32_t in_range = FULL ? static_cast<uint32_t>(tile_elems)
+                                 : static_cast<uint32_t>(p.n > base ? min(p.n - base, tile_elems) : 0);
 
-  // Both paths fill the same 16-byte vectors, converted to the work type
-  // after the join.  The scheduling barrier keeps the join's register copies
-  // behind the full path's loads, so all R loads are issued back to back
-  // (without it, gfx950 code issued the last load only once the first had
-  // returned; at 2^28 elements the difference was within run-to-run noise).
+  // The R loads are issued back to back, then converted to the work type.
   u32x4 q[R];
-  if (full) {
+  if constexpr (FULL) {
     const u32x4_a4 *src = reinterpret_cast<const u32x4_a4 *>(p.in + base);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -381,15 +376,14 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
       else
         q[r] = src[r * B + tid];
     }
-    __builtin_amdgcn_sched_barrier(0);
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       T tmp[VEC];
 #pragma unroll
       for (int k = 0; k < VEC; ++k) {
-        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
-        tmp[k] = (idx < p.n) ? p.in[idx] : T{};
+        const uint32_t i = static_cast<uint32_t>(r * B + tid) * VEC + k;
+        tmp[k] = (i < in_range) ? p.in[base + i] : T{};
       }
       __builtin_memcpy(&q[r], tmp, 16);
     }
@@ -402,51 +396,41 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
 #pragma unroll
     for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
   }
-  if constexpr (XF && VEC > 4) {
-    // Narrow elements: one uniform switch per row (a whole tile of widened
-    // 8- or 16-bit elements under one switch spilled to scratch).
-#pragma unroll
-    for (int r = 0; r < R; ++r) unary_detail::apply_row<T>(p.unary, v[r]);
-  } else if constexpr (XF) {
-    unary_detail::apply_tile<T>(p.unary, v);
-  }
-  if (!full) {
-    // Elements past the end act as the identity.
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-#pragma unroll
-      for (int k = 0; k < VEC; ++k) {
-        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
-        if (idx >= p.n) v[r][k] = Op::identity();
-      }
-    }
-  }
+  if constexpr (XF && VEC <= 4) unary_detail::apply_tile<T>(p.unary, v);
 
-  // Work-item-local inclusive scan of each row vector.
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-#pragma unroll
-    for (int k = 1; k < VEC; ++k) v[r][k] = Op::apply(v[r][k - 1], v[r][k]);
-  }
-  // Wave scans of the per-lane row totals.
+  // Per row: the work-item-local inclusive scan of the row vector, then the
+  // wave scan of the per-lane row totals.
   W wexcl[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
+    // Narrow elements: one uniform switch per row (a whole tile of widened
+    // 8- or 16-bit elements under one switch spilled to scratch).
+    if constexpr (XF && VEC > 4) unary_detail::apply_row<T>(p.unary, v[r]);
+    if constexpr (!FULL) {
+      // Elements past the end act as the identity.
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        const uint32_t i = static_cast<uint32_t>(r * B + tid) * VEC + k;
+        if (i >= in_range) v[r][k] = Op::identity();
+      }
+    }
+#pragma unroll
+    for (int k = 1; k < VEC; ++k) v[r][k] = Op::apply(v[r][k - 1], v[r][k]);
     const W incl = wave::inclusive_scan<Op>(v[r][VEC - 1]);
     wexcl[r] = wave::shift_up<Op>(incl);
-    if (lane == 63) s_wave[r * nw + w] = incl;
+    if (lane == 63) sh.wave[r * nw + w] = incl;
   }
   __syncthreads();
   // Wave 0 turns the (row, wave) totals into offsets and publishes the
   // tile's aggregate at once; with the one-wave look-back it then looks
   // back, while the other waves wait at the barrier.
   if (w == 0) {
-    const W agg = tile_prefixes<Op, W>(R * nw, s_wave);
+    const W agg = tile_prefixes<Op, W>(R * nw, sh.wave);
     if (tile == 0) {
       const W first = p.has_init ? p.init : Op::identity();
       if (lane == 0) {
         if (p.num_tiles > 1) SIO::publish(p.status, 0, epoch, st_prefix, Op::apply(first, agg));
-        s_excl = first;
+        sh.excl = first;
       }
     } else {
       if (lane == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
@@ -456,27 +440,27 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
         if (lane == 0) {
           if (tile + 1 < p.num_tiles)
             SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(excl, agg));
-          s_excl = excl;
+          sh.excl = excl;
         }
       }
     }
-    if (lane == 0) s_agg = agg;
+    if (lane == 0) sh.agg = agg;
   }
   __syncthreads();
   // Offsets of (row, wave) inside the tile, in element order.
   W before[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) before[r] = s_wave[r * nw + w];
+  for (int r = 0; r < R; ++r) before[r] = sh.wave[r * nw + w];
 
   W tile_prefix;
   if (tile != 0 && p.block_lookback) {
-    const W agg = s_agg;
-    tile_prefix = block_lookback<Op, W>(p.status, tile, epoch, s_masks, s_red, p.sync,
+    const W agg = sh.agg;
+    tile_prefix = block_lookback<Op, W>(p.status, tile, epoch, sh.masks, sh.red, p.sync,
                                        p.force_give_up != 0);
     if (tid == 0 && tile + 1 < p.num_tiles)
       SIO::publish(p.status, tile, epoch, st_prefix, Op::apply(tile_prefix, agg));
   } else {
-    tile_prefix = s_excl;
+    tile_prefix = sh.excl;
   }
 
 #pragma unroll
@@ -498,24 +482,49 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
       // every operator except the logical ones, which turn x into 0/1.
       if (r == 0 && tile == 0 && tid == 0 && !p.has_init) o[0] = static_cast<T>(v[0][0]);
     }
-    if (full) {
-      u32x4 q;
-      __builtin_memcpy(&q, o, 16);
+    if constexpr (FULL) {
+      u32x4 qo;
+      __builtin_memcpy(&qo, o, 16);
       // Ordinary stores by default: non-temporal output stores alone
       // measured no faster (380 us against 377 us at 2^28 int32).  The NT
       // variant streams both directions, as the fastest tile copy does.
       if constexpr (NT)
-        __builtin_nontemporal_store(q, reinterpret_cast<u32x4_a4 *>(p.out + base) + r * B + tid);
+        __builtin_nontemporal_store(qo, reinterpret_cast<u32x4_a4 *>(p.out + base) + r * B + tid);
       else
-        reinterpret_cast<u32x4_a4 *>(p.out + base)[r * B + tid] = q;
+        reinterpret_cast<u32x4_a4 *>(p.out + base)[r * B + tid] = qo;
     } else {
 #pragma unroll
       for (int k = 0; k < VEC; ++k) {
-        const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
-        if (idx < p.n) p.out[idx] = o[k];
+        const uint32_t i = static_cast<uint32_t>(r * B + tid) * VEC + k;
+        if (i < in_range) p.out[base + i] = o[k];
       }
     }
   }
+}
+
+template <class T, class Op, int R, bool XF = false, bool NT = false>
+__global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
+  using W = typename elem_traits<T>::work;
+  constexpr int VEC = 16 / sizeof(T);
+  __shared__ scan_shared<W, R> sh;
+
+  // Dynamic tile ids: tiles are handed out in the order work-groups start,
+  // so every predecessor of a tile is already running.  The counter is never
+  // reset; the host passes the value it had before this launch.
+  if (threadIdx.x == 0)
+    sh.tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+              p.ticket_base;
+  __syncthreads();
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(sh.tile);
+  // Tickets beyond this launch's tiles only appear if the host-side ticket
+  // count went out of step with the counter; never touch memory for them.
+  if (tile >= p.num_tiles) return;
+  const uint64_t tile_elems = static_cast<uint64_t>(blockDim.x) * R * VEC;
+  const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+  if (p.vec_io && base + tile_elems <= p.n)
+    scan_tile<true, T, Op, R, XF, NT>(p, tile, sh);
+  else
+    scan_tile<false, T, Op, R, XF, NT>(p, tile, sh);
 }
 
 }  // namespace scan_detail

@@ -59,7 +59,7 @@ def test_predicates_and_sizes(gpu, q, scratch, dtype):
             assert np.array_equal(out, ref), (n, pred)
 
 
-@pytest.mark.parametrize("dtype", [np.int32, np.float64])
+@pytest.mark.parametrize("dtype", TYPES)
 def test_stencil_and_misaligned(gpu, q, scratch, dtype):
     torch, _ = gpu
     rng = np.random.default_rng(2)
@@ -84,6 +84,27 @@ def test_stencil_and_misaligned(gpu, q, scratch, dtype):
         q.wait()
         c = int(cnt64.item())
         assert c == rc_st and np.array_equal(host(d)[:c], ref_st), soff
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("kind", ["all", "none", "runs"])
+def test_full_empty_and_run_selections(gpu, q, scratch, dtype, kind):
+    # Every element of every wave row kept (each row's LDS segment filled to
+    # its 64 x VEC capacity), none kept, and long alternating runs that
+    # start and end inside rows, waves and tiles; several full tiles plus a
+    # partial one.
+    torch, _ = gpu
+    n = (1 << 22) + 4099
+    i = np.arange(n)
+    if kind == "all":
+        x = (i % 50).astype(dtype)
+    elif kind == "none":
+        x = (i % 50 + 100).astype(dtype)
+    else:
+        x = np.where((i // 777) % 2 == 0, i % 50, 100).astype(dtype)
+    out, c = _run(torch, q, scratch, x, acpp.PRED_LT, dtype(60))
+    ref, rc = O.copy_if(x, O.PRED_LT, dtype(60))
+    assert c == rc and np.array_equal(out, ref)
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.int64])
