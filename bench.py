@@ -383,7 +383,7 @@ def tuner_report(tuner):
         pick = learned.choose(shape)
         times = tuner.autotune(shape, reps=5)
         best = min(times, key=times.get)
-        dflt = tuner.default_config(kind, shape.n)
+        dflt = tuner.default_config(kind, shape.n, dtype)
         key = lambda c: f"{c[0]}x{c[1]}"
         out[f"{kind}_{dtype}"] = {
             "learned_pick": key(pick), "learned_us": round(times[pick], 1),

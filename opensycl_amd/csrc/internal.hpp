@@ -66,10 +66,17 @@ static_assert(offsetof(sync_header, host_trip_flag) == 16, "trip flag pointer is
 // next synchronisation point (acpp_check_asynchronous_errors).  The flag is
 // written with a plain system-scope store (no read-modify-write on host
 // memory).
+// The flag is stored through a global-address-space pointer: a flat store
+// (the generic pointer) counts on two wait counters out of order, and in a
+// kernel that holds one anywhere the compiler waits for every outstanding
+// load where one would do.
 __device__ __forceinline__ void report_trip(sync_header *sync) {
   __hip_atomic_fetch_add(&sync->watchdog_trips, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t *flag = sync->host_trip_flag;
-  if (flag) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (flag) {
+    using global_u32 = __attribute__((address_space(1))) uint32_t;
+    __hip_atomic_store((global_u32 *)flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // Waiting inside a decoupled look-back.

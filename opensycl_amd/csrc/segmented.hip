@@ -358,7 +358,7 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
   const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
   const W id = Op::identity();
 
-  extern __shared__ uint32_t s_heads[];  // tile_elems + 1 bits
+  extern __shared__ uint32_t s_heads[];  // tile_elems + 1 bits, in 2 x blockDim.x words
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_tf[R * max_waves];  // per (row, wave): totals, then exclusive prefixes
   __shared__ W s_tv[R * max_waves];
@@ -394,243 +394,267 @@ __global__ __launch_bounds__(1024) void segmented_kernel(params<T> p) {
     if (tile >= p.num_tiles) return;  // queue drained
 
     const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
-    // Head bits of elements [base, base + tile_elems] (base is a multiple
-    // of 32).  Loaded before the tile's data so that waiting for them does
-    // not wait for the data (loads retire in order).
-    uint32_t hword0 = 0, hword1 = 0;
-    {
-      const uint64_t i0 = (base >> 5) + tid;
-      const uint64_t i1 = i0 + B;
-      if (static_cast<uint32_t>(tid) < head_words && i0 < p.head_words_total) hword0 = p.heads[i0];
-      if (static_cast<uint32_t>(tid + B) < head_words && i1 < p.head_words_total) hword1 = p.heads[i1];
-    }
-    __builtin_amdgcn_sched_barrier(0);
 
-    // Load the tile.
+    // Full tiles and the partial last tile as separate compile-time paths:
+    // with no join after the loads, each row waits only for its own load.
     const bool full = p.vec_io && (base + tile_elems <= p.total);
-    W v[R][VEC];
-    u32x4 q[R];
-    if (full) {
-      const u32x4_a4 *src = reinterpret_cast<const u32x4_a4 *>(p.in + base);
-#pragma unroll
-      for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
-      // Converted at once: the copies only rename registers (4- and 8-byte
-      // elements), and the loaded vectors then never live beside v.
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        T tmp[VEC];
-        __builtin_memcpy(tmp, &q[r], 16);
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+    auto tile_body = [&](auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
+      // elements of the tile in range, as a 32-bit offset inside the tile
+      const uint32_t in_range =
+          FULL ? static_cast<uint32_t>(tile_elems) : static_cast<uint32_t>(min(p.total - base, tile_elems));
+      // Head bits of elements [base, base + tile_elems] (base is a multiple
+      // of 32), loaded just before the tile's data in the same block, so that
+      // waiting for them does not wait for the data (loads retire in order).
+      // Both loads are unconditional (addresses clamped into the bitmap) and
+      // the words are masked only where they go to LDS, with opaque masks: a
+      // load under a branch, or a select the compiler turned back into one,
+      // made it wait for every outstanding load, the tile's data included,
+      // before the head bits could go to LDS.
+      uint32_t hword0, hword1, hmask0, hmask1;
+      {
+        const uint64_t last = p.head_words_total - 1;
+        const uint64_t i0 = (base >> 5) + tid;
+        const uint64_t i1 = i0 + B;
+        hword0 = p.heads[i0 < last ? i0 : last];
+        hword1 = p.heads[i1 < last ? i1 : last];
+        hmask0 = (static_cast<uint32_t>(tid) < head_words && i0 <= last) ? ~0u : 0u;
+        hmask1 = (static_cast<uint32_t>(tid + B) < head_words && i1 <= last) ? ~0u : 0u;
+        asm volatile("" : "+v"(hmask0), "+v"(hmask1));
       }
-    } else {
+      __builtin_amdgcn_sched_barrier(0);
+      W v[R][VEC];
+      u32x4 q[R];
+      if constexpr (FULL) {
+        const u32x4_a4 *src = reinterpret_cast<const u32x4_a4 *>(p.in + base);
+#pragma unroll
+        for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+        // Converted at once: the copies only rename registers (4- and 8-byte
+        // elements), and the loaded vectors then never live beside v.
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          T tmp[VEC];
+          __builtin_memcpy(tmp, &q[r], 16);
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) {
+            const uint32_t li = static_cast<uint32_t>(r * B + tid) * VEC + k;
+            v[r][k] = (li < in_range) ? elem_traits<T>::to_work(p.in[base + li]) : id;
+          }
+      }
+      // Unconditional stores (2 B words of LDS; the words past head_words
+      // are zero): a store under a branch made the compiler wait for the
+      // tile's data as well.
+      s_heads[tid] = hword0 & hmask0;
+      s_heads[tid + B] = hword1 & hmask1;
+      __syncthreads();
+
+      auto head_at = [&](uint64_t pos) -> uint32_t {
+        return (s_heads[pos >> 5] >> (pos & 31)) & 1u;
+      };
+
+      // Head bits of the work-item's elements, all rows in one word: bit
+      // r * VEC + k = element k of row r starts a segment.
+      const uint32_t fm = row_head_bits<R, VEC>(s_heads, tid, B);
+
+      // Work-item-local segmented scan of each row vector: an element that
+      // starts a segment keeps its value, the others add the running value.
 #pragma unroll
       for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int k = 0; k < VEC; ++k) {
-          const uint64_t idx = base + static_cast<uint64_t>(r * B + tid) * VEC + k;
-          v[r][k] = (idx < p.total) ? elem_traits<T>::to_work(p.in[idx]) : id;
-        }
-    }
-    if (static_cast<uint32_t>(tid) < head_words) s_heads[tid] = hword0;
-    if (static_cast<uint32_t>(tid + B) < head_words) s_heads[tid + B] = hword1;
-    __syncthreads();
+        for (int k = 1; k < VEC; ++k)
+          v[r][k] = ((fm >> (r * VEC + k)) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
 
-    auto head_at = [&](uint64_t pos) -> uint32_t {
-      return (s_heads[pos >> 5] >> (pos & 31)) & 1u;
-    };
-
-    // Head bits of the work-item's elements, all rows in one word: bit
-    // r * VEC + k = element k of row r starts a segment.
-    const uint32_t fm = row_head_bits<R, VEC>(s_heads, tid, B);
-
-    // Work-item-local segmented scan of each row vector: an element that
-    // starts a segment keeps its value, the others add the running value.
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-      for (int k = 1; k < VEC; ++k)
-        v[r][k] = ((fm >> (r * VEC + k)) & 1u) ? v[r][k] : Op::apply(v[r][k - 1], v[r][k]);
-
-    // Wave scans of the row totals; the wave-exclusive prefix is folded into
-    // the row's elements at once (up to the row's first segment start), so
-    // only one bit per row stays live: bit r of xcut = a segment starts in
-    // an earlier lane of this wave's row r.
-    const uint32_t xcut = fold_wave_prefixes<Op, R, VEC>(fm, v, s_tf, s_tv, nw, w, lane);
-    __syncthreads();
-    // Wave 0 turns the (row, wave) totals into exclusive prefixes and
-    // publishes the tile's aggregate at once.  A tile that contains a
-    // segment start publishes it as its prefix: it is final for everything
-    // after the tile.
-    if (w == 0) {
-      uint32_t af;
-      W av;
-      tile_pair_prefixes<Op, W>(R * nw, s_tf, s_tv, af, av);
-      if (lane == 0) {
-        s_aggf = af;
-        s_aggv = av;
-        if (tile == 0) {
-          if (p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, af, av);
-        } else {
-          SIO::publish(p.status, tile, p.epoch, af ? st_prefix : st_aggregate, af, av);
-        }
-      }
-    }
-    __syncthreads();
-    // s_tf / s_tv now hold the pairs before each (row, wave) inside the
-    // tile, in element order; they are read again for the results (the
-    // next tile rewrites them only after the barrier at the top of the loop).
-    const uint32_t agg_f = s_aggf;
-    const W agg_v = s_aggv;
-
-    // Tile prefix: ordered look-back (one wave, or the whole work-group).
-    uint32_t pf = 0;
-    W pv = id;
-    if (tile != 0) {
-      // Only elements before the tile's first segment start need the
-      // prefix; a tile that begins with a segment start needs none.
-      if (!head_at(0) && !p.block_lookback) {
-        // One wave looks back; its first lane publishes the tile's prefix.
-        if (w == 0) {
-          uint32_t ef;
-          W ev;
-          wait_own_stores();  // the aggregate is out before the first poll
-          wave_pair_lookback<Op, W>(p.status, tile, p.epoch, p.sync, p.force_give_up != 0, ef, ev);
-          if (lane == 0) {
-            s_lbf[0] = ef;
-            s_lbv[0] = ev;
-            if (!agg_f && tile + 1 < p.num_tiles) {
-              uint32_t of = agg_f;
-              W ov = agg_v;
-              combine<Op>(ef, ev, of, ov);
-              SIO::publish(p.status, tile, p.epoch, st_prefix, of, ov);
-            }
+      // Wave scans of the row totals; the wave-exclusive prefix is folded into
+      // the row's elements at once (up to the row's first segment start), so
+      // only one bit per row stays live: bit r of xcut = a segment starts in
+      // an earlier lane of this wave's row r.
+      const uint32_t xcut = fold_wave_prefixes<Op, R, VEC>(fm, v, s_tf, s_tv, nw, w, lane);
+      __syncthreads();
+      // Wave 0 turns the (row, wave) totals into exclusive prefixes and
+      // publishes the tile's aggregate at once.  A tile that contains a
+      // segment start publishes it as its prefix: it is final for everything
+      // after the tile.
+      if (w == 0) {
+        uint32_t af;
+        W av;
+        tile_pair_prefixes<Op, W>(R * nw, s_tf, s_tv, af, av);
+        if (lane == 0) {
+          s_aggf = af;
+          s_aggv = av;
+          if (tile == 0) {
+            if (p.num_tiles > 1) SIO::publish(p.status, 0, p.epoch, st_prefix, af, av);
+          } else {
+            SIO::publish(p.status, tile, p.epoch, af ? st_prefix : st_aggregate, af, av);
           }
         }
-        __syncthreads();
-        pf = s_lbf[0];
-        pv = s_lbv[0];
-      } else if (!head_at(0)) {
-        // thread t polls tile (base_t - (B - 1 - t)); thread B-1 = nearest.
-        int64_t wbase = static_cast<int64_t>(tile) - 1;
-        uint32_t ef = 0;  // accumulated prefix of tiles nearer than the window
-        W ev = id;
-        lookback_wait wait;
-        wait.begin();
-        bool give_up = p.force_give_up != 0;  // uniform across the work-group
-        if (give_up && tid == 0) report_trip(p.sync);
-        for (;;) {
-          const int64_t j = wbase - (B - 1 - tid);
-          uint32_t lf = 0;
-          W lv = id;
-          int q;
-          for (;;) {
-            uint32_t st = st_prefix;
-            lf = 0;
-            lv = id;
-            if (j >= 0) st = SIO::read(p.status, static_cast<uint32_t>(j), p.epoch, lf, lv);
-            if (give_up) st = st_prefix;
-            const bool term = (st == st_prefix) || (st == st_aggregate && lf);
-            const uint64_t tm = wave::ballot(term);
-            const uint64_t im = wave::ballot(st == st_invalid);
+      }
+      __syncthreads();
+      // s_tf / s_tv now hold the pairs before each (row, wave) inside the
+      // tile, in element order; they are read again for the results (the
+      // next tile rewrites them only after the barrier at the top of the loop).
+      const uint32_t agg_f = s_aggf;
+      const W agg_v = s_aggv;
+
+      // Tile prefix: ordered look-back (one wave, or the whole work-group).
+      uint32_t pf = 0;
+      W pv = id;
+      if (tile != 0) {
+        // Only elements before the tile's first segment start need the
+        // prefix; a tile that begins with a segment start needs none.
+        if (!head_at(0) && !p.block_lookback) {
+          // One wave looks back; its first lane publishes the tile's prefix.
+          if (w == 0) {
+            uint32_t ef;
+            W ev;
+            wait_own_stores();  // the aggregate is out before the first poll
+            wave_pair_lookback<Op, W>(p.status, tile, p.epoch, p.sync, p.force_give_up != 0, ef, ev);
             if (lane == 0) {
-              s_q[w] = tm ? (w * 64 + 63 - __builtin_clzll(tm)) : -1;
-              s_invalid[w] = im;
-            }
-            __syncthreads();
-            q = -1;
-            for (int ww = nw - 1; ww >= 0; --ww)
-              if (s_q[ww] >= 0) {
-                q = s_q[ww];
-                break;
+              s_lbf[0] = ef;
+              s_lbv[0] = ev;
+              if (!agg_f && tile + 1 < p.num_tiles) {
+                uint32_t of = agg_f;
+                W ov = agg_v;
+                combine<Op>(ef, ev, of, ov);
+                SIO::publish(p.status, tile, p.epoch, st_prefix, of, ov);
               }
-            bool bad = false;
+            }
+          }
+          __syncthreads();
+          pf = s_lbf[0];
+          pv = s_lbv[0];
+        } else if (!head_at(0)) {
+          // thread t polls tile (base_t - (B - 1 - t)); thread B-1 = nearest.
+          int64_t wbase = static_cast<int64_t>(tile) - 1;
+          uint32_t ef = 0;  // accumulated prefix of tiles nearer than the window
+          W ev = id;
+          lookback_wait wait;
+          wait.begin();
+          bool give_up = p.force_give_up != 0;  // uniform across the work-group
+          if (give_up && tid == 0) report_trip(p.sync);
+          for (;;) {
+            const int64_t j = wbase - (B - 1 - tid);
+            uint32_t lf = 0;
+            W lv = id;
+            int q;
+            for (;;) {
+              uint32_t st = st_prefix;
+              lf = 0;
+              lv = id;
+              if (j >= 0) st = SIO::read(p.status, static_cast<uint32_t>(j), p.epoch, lf, lv);
+              if (give_up) st = st_prefix;
+              const bool term = (st == st_prefix) || (st == st_aggregate && lf);
+              const uint64_t tm = wave::ballot(term);
+              const uint64_t im = wave::ballot(st == st_invalid);
+              if (lane == 0) {
+                s_q[w] = tm ? (w * 64 + 63 - __builtin_clzll(tm)) : -1;
+                s_invalid[w] = im;
+              }
+              __syncthreads();
+              q = -1;
+              for (int ww = nw - 1; ww >= 0; --ww)
+                if (s_q[ww] >= 0) {
+                  q = s_q[ww];
+                  break;
+                }
+              bool bad = false;
+              for (int ww = 0; ww < nw; ++ww) {
+                const int lo = ww * 64;
+                uint64_t need;
+                if (q <= lo) need = ~0ull;
+                else if (q > lo + 63) need = 0;
+                else need = ~0ull << (q - lo);
+                bad |= (s_invalid[ww] & need) != 0;
+              }
+              __syncthreads();
+              if (!bad) break;
+              // Every work-item backs off; work-item 0's clock decides for all.
+              const bool late = wait.failed_round();
+              if (tid == 0) {
+                s_give_up = late ? 1 : 0;
+                if (late) report_trip(p.sync);
+              }
+              __syncthreads();
+              give_up = s_give_up != 0;
+            }
+            // Fold window tiles [q .. B-1] in tile order.
+            uint32_t xf2 = (tid >= q) ? lf : 0;
+            W xv2 = (tid >= q) ? lv : id;
+            wave_pair_scan<Op>(xf2, xv2);
+            if (lane == 63) {
+              s_lbf[w] = xf2;
+              s_lbv[w] = xv2;
+            }
+            __syncthreads();
+            uint32_t wf = 0;
+            W wv = id;
             for (int ww = 0; ww < nw; ++ww) {
-              const int lo = ww * 64;
-              uint64_t need;
-              if (q <= lo) need = ~0ull;
-              else if (q > lo + 63) need = 0;
-              else need = ~0ull << (q - lo);
-              bad |= (s_invalid[ww] & need) != 0;
+              uint32_t yf = s_lbf[ww];
+              W yv = s_lbv[ww];
+              combine<Op>(wf, wv, yf, yv);
+              wf = yf;
+              wv = yv;
             }
             __syncthreads();
-            if (!bad) break;
-            // Every work-item backs off; work-item 0's clock decides for all.
-            const bool late = wait.failed_round();
-            if (tid == 0) {
-              s_give_up = late ? 1 : 0;
-              if (late) report_trip(p.sync);
-            }
-            __syncthreads();
-            give_up = s_give_up != 0;
+            // window (earlier) . accumulated (later)
+            combine<Op>(wf, wv, ef, ev);
+            if (q >= 0) break;
+            wbase -= B;
           }
-          // Fold window tiles [q .. B-1] in tile order.
-          uint32_t xf2 = (tid >= q) ? lf : 0;
-          W xv2 = (tid >= q) ? lv : id;
-          wave_pair_scan<Op>(xf2, xv2);
-          if (lane == 63) {
-            s_lbf[w] = xf2;
-            s_lbv[w] = xv2;
+          pf = ef;
+          pv = ev;
+          if (tid == 0 && !agg_f && tile + 1 < p.num_tiles) {
+            uint32_t of = agg_f;
+            W ov = agg_v;
+            combine<Op>(pf, pv, of, ov);
+            SIO::publish(p.status, tile, p.epoch, st_prefix, of, ov);
           }
-          __syncthreads();
-          uint32_t wf = 0;
-          W wv = id;
-          for (int ww = 0; ww < nw; ++ww) {
-            uint32_t yf = s_lbf[ww];
-            W yv = s_lbv[ww];
-            combine<Op>(wf, wv, yf, yv);
-            wf = yf;
-            wv = yv;
-          }
-          __syncthreads();
-          // window (earlier) . accumulated (later)
-          combine<Op>(wf, wv, ef, ev);
-          if (q >= 0) break;
-          wbase -= B;
-        }
-        pf = ef;
-        pv = ev;
-        if (tid == 0 && !agg_f && tile + 1 < p.num_tiles) {
-          uint32_t of = agg_f;
-          W ov = agg_v;
-          combine<Op>(pf, pv, of, ov);
-          SIO::publish(p.status, tile, p.epoch, st_prefix, of, ov);
         }
       }
-    }
 
-    // Results: the elements already hold the fold from the start of their
-    // wave's row (or of their segment); the rest is P = tile prefix . before
-    // (row r, wave w), cut off by a segment start in an earlier lane or
-    // among elements 0..k of the vector.
-    (void)pf;
+      // Results: the elements already hold the fold from the start of their
+      // wave's row (or of their segment); the rest is P = tile prefix . before
+      // (row r, wave w), cut off by a segment start in an earlier lane or
+      // among elements 0..k of the vector.
+      (void)pf;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const uint32_t bfr = s_tf[r * nw + w];
-      const W bvr = s_tv[r * nw + w];
-      const W P = bfr ? bvr : Op::apply(pv, bvr);
-      const uint32_t rowf = (fm >> (r * VEC)) & ((1u << VEC) - 1u);
-      const bool wcut = (xcut >> r) & 1u;
-      W o[VEC];
+      for (int r = 0; r < R; ++r) {
+        const uint32_t bfr = s_tf[r * nw + w];
+        const W bvr = s_tv[r * nw + w];
+        const W P = bfr ? bvr : Op::apply(pv, bvr);
+        const uint32_t rowf = (fm >> (r * VEC)) & ((1u << VEC) - 1u);
+        const bool wcut = (xcut >> r) & 1u;
+        W o[VEC];
 #pragma unroll
-      for (int k = 0; k < VEC; ++k) {
-        const bool cut = wcut || (rowf & ((2u << k) - 1u)) != 0u;
-        o[k] = cut ? v[r][k] : Op::apply(P, v[r][k]);
+        for (int k = 0; k < VEC; ++k) {
+          const bool cut = wcut || (rowf & ((2u << k) - 1u)) != 0u;
+          o[k] = cut ? v[r][k] : Op::apply(P, v[r][k]);
+        }
+        if constexpr (FULL) {
+          T ot[VEC];
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) ot[k] = static_cast<T>(o[k]);
+          u32x4 qv;
+          __builtin_memcpy(&qv, ot, 16);
+          reinterpret_cast<u32x4_a4 *>(p.out + base)[r * B + tid] = qv;
+        } else {
+#pragma unroll
+          for (int k = 0; k < VEC; ++k) {
+            const uint32_t li = static_cast<uint32_t>(r * B + tid) * VEC + k;
+            if (li < in_range) p.out[base + li] = static_cast<T>(o[k]);
+          }
+        }
       }
-      const uint64_t e0 = base + static_cast<uint64_t>(r * B + tid) * VEC;
-      if (full) {
-        T ot[VEC];
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) ot[k] = static_cast<T>(o[k]);
-        u32x4 qv;
-        __builtin_memcpy(&qv, ot, 16);
-        reinterpret_cast<u32x4_a4 *>(p.out + base)[r * B + tid] = qv;
-      } else {
-#pragma unroll
-        for (int k = 0; k < VEC; ++k)
-          if (e0 + k < p.total) p.out[e0 + k] = static_cast<T>(o[k]);
-      }
-    }
+    };
+    if (full)
+      tile_body(std::true_type{});
+    else
+      tile_body(std::false_type{});
   }
 }
 
@@ -838,21 +862,23 @@ __global__ void segmented_fixup_kernel(params<T> p, const typename elem_traits<T
 // Host side
 // --------------------------------------------------------------------------
 // Default tile when the caller gives no configuration, by total element
-// count.  Measured on MI355X over 500 segmented shapes up to 2^28 elements
-// (the tuner's timing database): this rule has a mean regret of ~7 %
-// against the best of the 15 configurations, where any single fixed tile
-// has 15-70 %.
-//   scan:   n < 2^16: 256 x 2;  n < 2^20: 512 x 2;  otherwise 512 x 8
+// count and element size.  Read off the first 400 of 500 segmented shapes
+// up to 2^28 elements timed on MI355X (the tuner's timing database); on the
+// other 100 it has a mean regret of ~4 % against the best of the 15
+// configurations, where any single fixed tile has 15-70 %.
+//   scan:   n < 2^16: 256 x 2;  n < 2^20: 512 x 2 (8-byte: 1024 x 2);
+//           n < 2^26: 512 x 8;  otherwise 1024 x 8
 //   reduce: n < 2^16: 128 x 2;  n < 2^20: 256 x 2;  n < 2^24: 1024 x 2;
-//           otherwise 256 x 4
-// Large scans take 8-row tiles, two 512-wide groups per CU (126 VGPRs for
-// int64); large reductions take 4-row tiles, whose 60 VGPRs leave room for
-// several groups per CU (the reduction's groups never wait for each other).
-inline uint32_t default_work_group(bool reduce, uint64_t total) {
+//           otherwise 256 x 4 (8-byte: 512 x 4)
+// Large scans take 8-row tiles (two 512-wide or one 1024-wide group per CU,
+// 123 VGPRs for int64); large reductions take 4-row tiles, which leave room
+// for several groups per CU (the reduction's groups never wait for each
+// other).
+inline uint32_t default_work_group(bool reduce, uint64_t total, int elem_bytes) {
   if (total < (1ull << 16)) return reduce ? 128 : 256;
-  if (total < (1ull << 20)) return reduce ? 256 : 512;
-  if (reduce && total < (1ull << 24)) return 1024;
-  return reduce ? 256 : 512;
+  if (total < (1ull << 20)) return reduce ? 256 : (elem_bytes == 8 ? 1024 : 512);
+  if (reduce) return total < (1ull << 24) ? 1024 : (elem_bytes == 8 ? 512 : 256);
+  return total < (1ull << 26) ? 512 : 1024;
 }
 inline int default_rows(bool reduce, uint64_t total) {
   if (total < (1ull << 20)) return 2;
@@ -959,7 +985,7 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   acpp_status st = enter_device(g, &cus);
   if (st != ACPP_SUCCESS) return st;
   launch_plan<T, Op> lp;
-  lp.wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : default_work_group(reduce, total);
+  lp.wg = (cfg && cfg->work_group_size) ? cfg->work_group_size : default_work_group(reduce, total, static_cast<int>(sizeof(T)));
   if (!valid_work_group_size(lp.wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad work_group_size");
   lp.rows = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows(reduce, total);
@@ -969,7 +995,9 @@ acpp_status enqueue(acpp_allocation_group *g, hipStream_t stream, const T *in, T
   const uint64_t num_tiles = std::max<uint64_t>(1, ceil_div(total, lp.tile_elems));
   if (num_tiles >= (1ull << 31) || num_segments >= (1ull << 32))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT, std::string(name) + ": problem too large");
-  lp.lds = static_cast<size_t>((lp.tile_elems + 1 + 31) / 32) * 4;
+  // the scan kernel stores 2 x wg words of head bits (>= the tile_elems + 1
+  // bits it needs, since rows x 16-byte vector <= 32 elements)
+  lp.lds = std::max<size_t>(static_cast<size_t>((lp.tile_elems + 1 + 31) / 32), 2 * static_cast<size_t>(lp.wg)) * 4;
 
   void *ts = nullptr;
   st = group_obtain(g, (num_tiles + 1) * sizeof(uint32_t), &ts);

@@ -50,16 +50,19 @@ CONFIGS = [(wg, r) for wg in WORK_GROUP_SIZES for r in ROWS]
 DEFAULT_CONFIG = (512, 4)
 
 
-def default_config(kind: str, n: int) -> tuple:
+def default_config(kind: str, n: int, dtype: str = "i32") -> tuple:
     """The library's own choice when no configuration is given: the size
-    rule of segmented.hip (default_work_group / default_rows)."""
+    and element-size rule of segmented.hip (default_work_group /
+    default_rows)."""
+    wide = DTYPES.get(dtype, 4) == 8
     if n < (1 << 16):
         return (128, 2) if kind == "reduce" else (256, 2)
     if n < (1 << 20):
-        return (256, 2) if kind == "reduce" else (512, 2)
+        return (256, 2) if kind == "reduce" else ((1024, 2) if wide else (512, 2))
     if kind == "reduce":
-        return (1024, 2) if n < (1 << 24) else (256, 4)
-    return (512, 8)
+        return (1024, 2) if n < (1 << 24) else ((512, 4) if wide else (256, 4))
+    return (512, 8) if n < (1 << 26) else (1024, 8)
+
 
 KINDS = ("scan", "reduce")
 DTYPES = {"i32": 4, "i64": 8, "f32": 4, "f64": 8}
@@ -387,7 +390,8 @@ def regret(db: list, picks: list) -> dict:
         best = min(t.values())
         tp = t[_key(pick)]
         sh = rec.get("shape", {})
-        dflt = default_config(sh["kind"], sh["n"]) if "kind" in sh and "n" in sh else DEFAULT_CONFIG
+        dflt = (default_config(sh["kind"], sh["n"], sh.get("dtype", "i32"))
+                if "kind" in sh and "n" in sh else DEFAULT_CONFIG)
         td = t[_key(dflt)]
         tf = t[_key(DEFAULT_CONFIG)]
         r.append(tp / best - 1.0)

@@ -104,10 +104,13 @@ def test_library_default_rule_on_committed_timings():
     # regret below 10 %, and better than the former single fixed tile.
     db = T.load_db(os.path.join(T.DATA_DIR, "segmented_timings.json"))
     held = db[400:]
-    picks = [T.default_config(r["shape"]["kind"], r["shape"]["n"]) for r in held]
+    picks = [T.default_config(r["shape"]["kind"], r["shape"]["n"], r["shape"]["dtype"]) for r in held]
     r = T.regret(held, picks)
     assert r["mean_regret"] < 0.10, r
     assert r["mean_regret"] < r["fixed_512x4_mean_regret"], r
-    assert T.default_config("scan", 1 << 28) == (512, 8)
-    assert T.default_config("reduce", 1 << 28) == (256, 4)
+    assert T.default_config("scan", 1 << 28, "i64") == (1024, 8)
+    assert T.default_config("scan", 1 << 24, "i32") == (512, 8)
+    assert T.default_config("scan", 1 << 18, "i64") == (1024, 2)
+    assert T.default_config("reduce", 1 << 28, "f32") == (256, 4)
+    assert T.default_config("reduce", 1 << 28, "f64") == (512, 4)
     assert T.default_config("reduce", 1000) == (128, 2)
