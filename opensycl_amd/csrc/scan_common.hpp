@@ -527,5 +527,163 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
     scan_tile<false, T, Op, R, XF, NT>(p, tile, sh);
 }
 
+// --------------------------------------------------------------------------
+// 8-bit plus scans on packed 16-bit pairs.
+//
+// The generic kernel widens every 8-bit element to a 32-bit work value, so
+// a 16-element vector takes 16 registers and only 4-row tiles fit.  For
+// plus, the 8-bit result is the sum modulo 256, so any wider modular sum
+// will do: register j of a row holds element j in its low and element
+// j + 8 in its high 16 bits (one v_perm_b32 each), and the row's
+// inclusive scan runs on both halves at once with packed 16-bit adds
+// (v_pk_add_u16), the high half then offset by the low half's total.  A row
+// takes 8 registers, so 8-row (128 KiB) tiles fit, with about half the
+// instructions per element.  Row totals, tile aggregates and prefixes are
+// 32-bit sums (exact modulo 2^32, hence modulo 256), and the output keeps
+// each sum's low byte.  2^28 int8: 109-111 us against 125-133 us for the
+// generic 4-row kernel (MI355X, same runs).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+
+constexpr int packed8_rows = 8;
+
+template <bool FULL, class T, int R>
+__device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, uint32_t *s_wave, uint32_t &s_excl) {
+  static_assert(sizeof(T) == 1, "packed 8-bit scan");
+  using Op = ops::plus<uint32_t, uint32_t>;
+  using SIO = status_io<uint32_t>;
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int nw = B >> 6;
+  const int w = tid >> 6;
+  const int lane = tid & 63;
+  const uint32_t epoch = p.epoch;
+  const uint64_t tile_elems = static_cast<uint64_t>(B) * R * 16;
+  const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+  const uint32_t in_range = FULL ? static_cast<uint32_t>(tile_elems)
+                                 : static_cast<uint32_t>(p.n > base ? min(p.n - base, tile_elems) : 0);
+  const uint8_t *in8 = reinterpret_cast<const uint8_t *>(p.in) + base;
+  uint8_t *out8 = reinterpret_cast<uint8_t *>(p.out) + base;
+
+  u32x4 q[R];
+  if constexpr (FULL) {
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(in8);
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+  } else {
+    // Elements past the end read as 0, the identity of plus.
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      uint32_t d[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint32_t i = static_cast<uint32_t>(r * B + tid) * 16 + k;
+        if (i < in_range) d[k >> 2] |= static_cast<uint32_t>(in8[i]) << (8 * (k & 3));
+      }
+      q[r] = u32x4{d[0], d[1], d[2], d[3]};
+    }
+  }
+  u16x2 P[R][8];
+  uint32_t wexcl[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t d[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
+    // pair j = (byte j, byte j + 8), zero-extended to 16 bits each
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t sel = (j & 3) | (12u << 8) | ((4u + (j & 3)) << 16) | (12u << 24);
+      P[r][j] = as_u16x2(__builtin_amdgcn_perm(d[(j >> 2) + 2], d[j >> 2], sel));
+    }
+#pragma unroll
+    for (int j = 1; j < 8; ++j) P[r][j] = P[r][j - 1] + P[r][j];
+    // the high half continues after element 7
+    const u16x2 fix = {0, P[r][7].x};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) P[r][j] = P[r][j] + fix;
+    const uint32_t incl = wave::inclusive_scan<Op>(static_cast<uint32_t>(P[r][7].y));
+    wexcl[r] = wave::shift_up<Op>(incl);
+    if (lane == 63) s_wave[r * nw + w] = incl;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const uint32_t agg = tile_prefixes<Op, uint32_t>(R * nw, s_wave);
+    if (tile == 0) {
+      const uint32_t first = p.has_init ? static_cast<uint32_t>(p.init) : 0u;
+      if (lane == 0) {
+        if (p.num_tiles > 1) SIO::publish(p.status, 0, epoch, st_prefix, first + agg);
+        s_excl = first;
+      }
+    } else {
+      if (lane == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
+      wait_own_stores();  // the aggregate is out before the first poll
+      const uint32_t excl = lookback<Op, uint32_t, lookback_slots>(p.status, tile, epoch, p.sync,
+                                                                   p.force_give_up != 0);
+      if (lane == 0) {
+        if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, epoch, st_prefix, excl + agg);
+        s_excl = excl;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t tile_prefix = s_excl;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t P32 = tile_prefix + s_wave[r * nw + w] + wexcl[r];
+    const unsigned short p16 = static_cast<unsigned short>(P32);
+    const u16x2 pv = {p16, p16};
+    u16x2 O[8];
+    if (p.exclusive) {
+      // element j takes the inclusive value of element j - 1; element 8
+      // (high half of pair 0) that of element 7
+      const u16x2 e0 = {0, P[r][7].x};
+      O[0] = e0 + pv;
+#pragma unroll
+      for (int j = 1; j < 8; ++j) O[j] = P[r][j - 1] + pv;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) O[j] = P[r][j] + pv;
+    }
+    // low bytes back in element order: [e0 e1 e8 e9] and [e2 e3 e10 e11]
+    // give dwords 0 and 2; likewise pairs 4..7 give dwords 1 and 3
+    const uint32_t a0 = __builtin_amdgcn_perm(as_u32(O[1]), as_u32(O[0]), 0x06020400u);
+    const uint32_t a1 = __builtin_amdgcn_perm(as_u32(O[3]), as_u32(O[2]), 0x06020400u);
+    const uint32_t a2 = __builtin_amdgcn_perm(as_u32(O[5]), as_u32(O[4]), 0x06020400u);
+    const uint32_t a3 = __builtin_amdgcn_perm(as_u32(O[7]), as_u32(O[6]), 0x06020400u);
+    const uint32_t o[4] = {__builtin_amdgcn_perm(a1, a0, 0x05040100u), __builtin_amdgcn_perm(a3, a2, 0x05040100u),
+                           __builtin_amdgcn_perm(a1, a0, 0x07060302u), __builtin_amdgcn_perm(a3, a2, 0x07060302u)};
+    if constexpr (FULL) {
+      reinterpret_cast<u32x4 *>(out8)[r * B + tid] = u32x4{o[0], o[1], o[2], o[3]};
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint32_t i = static_cast<uint32_t>(r * B + tid) * 16 + k;
+        if (i < in_range) out8[i] = static_cast<uint8_t>(o[k >> 2] >> (8 * (k & 3)));
+      }
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void scan8_kernel(params<T> p) {
+  constexpr int R = packed8_rows;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wave[R * max_waves];
+  __shared__ uint32_t s_excl;
+  if (threadIdx.x == 0)
+    s_tile = __hip_atomic_fetch_add(&p.sync->tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+             p.ticket_base;
+  __syncthreads();
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+  if (tile >= p.num_tiles) return;
+  const uint64_t tile_elems = static_cast<uint64_t>(blockDim.x) * R * 16;
+  const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
+  if (p.vec_io && base + tile_elems <= p.n)
+    scan8_tile<true, T, R>(p, tile, s_wave, s_excl);
+  else
+    scan8_tile<false, T, R>(p, tile, s_wave, s_excl);
+}
+
 }  // namespace scan_detail
 }  // namespace acpp_impl

@@ -7,13 +7,14 @@
 namespace acpp_impl {
 namespace scan_detail {
 
-// Default rows (16-byte vectors) per work-item: 8, or 4 for 1-byte
-// elements (16 per vector, widened to 32 bits: 8 rows would not fit the
-// registers).  Fewer rows mean more tiles per byte, and the look-back
-// chain, not the memory, then bounds the scan: 2^28 int8 took 153 us with
-// 2 rows.  8-row tiles of 1-byte elements kept packed in registers (and
-// unpacked twice) measured slower than these 4-row tiles: 145-151 us
-// against 127-134 us at 2^28 int8 and uint8.
+// Default rows (16-byte vectors) per work-item of the generic kernel: 8,
+// or 4 for 1-byte elements (16 per vector, widened to 32 bits: 8 rows would
+// not fit the registers).  Fewer rows mean more tiles per byte, and the
+// look-back chain, not the memory, then bounds the scan: 2^28 int8 took
+// 153 us with 2 rows.  8-row tiles of 1-byte elements kept as raw bytes in
+// registers (and unpacked twice) measured slower than these 4-row tiles:
+// 145-151 us against 127-134 us at 2^28 int8 and uint8.  The 8-bit plus
+// scans take 8-row tiles on packed 16-bit pairs instead (scan8_kernel).
 template <class T>
 constexpr int default_rows() {
   return sizeof(T) >= 2 ? 8 : 4;
@@ -38,6 +39,13 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     if (st != ACPP_SUCCESS) return st;
   }
 
+  // 8-bit plus scans run the packed-pair kernel (8 rows) unless the caller
+  // asks for 4 rows or the work-group-wide look-back, which only the
+  // generic kernel has.
+  constexpr bool packed8 = sizeof(T) == 1 && Op::id == ops::op_plus && !XF;
+  const bool block_lb = cfg && (cfg->flags & variant_block_lookback);
+  const int dflt_rows = (packed8 && !block_lb) ? packed8_rows : default_rows<T>();
+
   // Default work-group size: 1024-wide tiles (fewest tiles, so the fewest
   // look-back links) when the input still gives every CU a few tiles,
   // otherwise 256-wide tiles for parallelism on small inputs.
@@ -45,20 +53,24 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
   if (cfg && cfg->work_group_size) {
     wg = cfg->work_group_size;
   } else {
-    const int rows0 = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows<T>();
+    const int rows0 = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : dflt_rows;
     if (ceil_div(n, static_cast<uint64_t>(1024) * rows0 * VEC) >= 4ull * static_cast<uint64_t>(cus)) wg = 1024;
   }
   if (!valid_work_group_size(wg))
     return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                      std::string(name) + ": work_group_size must be a multiple of 64 in [64, 1024]");
-  int rows = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : default_rows<T>();
+  int rows = (cfg && cfg->items_per_thread) ? static_cast<int>(cfg->items_per_thread) : dflt_rows;
   constexpr bool tunable = (Op::id == ops::op_plus) && sizeof(T) >= 4 && !XF;
-  if (rows != default_rows<T>()) {
-    if (!tunable || (rows != 2 && rows != 4 && rows != 8))
+  if (rows != dflt_rows) {
+    const bool ok = tunable ? (rows == 2 || rows == 4 || rows == 8)
+                            : (packed8 && rows == default_rows<T>());
+    if (!ok)
       return set_error(ACPP_ERROR_INVALID_ARGUMENT,
                        std::string(name) + ": items_per_thread (rows per work-item) must be " +
-                           std::to_string(default_rows<T>()) +
-                           (tunable ? ", 2 or 4" : " for this element type and operator"));
+                           std::to_string(dflt_rows) +
+                           (tunable ? ", 2 or 4"
+                                    : packed8 ? (block_lb ? " with the work-group look-back" : " or 4")
+                                              : " for this element type and operator"));
   }
 
   const uint64_t tile_elems = static_cast<uint64_t>(wg) * rows * VEC;
@@ -120,6 +132,9 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     end_region_launch(rs, rl, static_cast<uint32_t>(num_tiles), s);
     return s;
   };
+  if constexpr (packed8) {
+    if (rows == packed8_rows) return run(scan8_kernel<T>);
+  }
   if constexpr (tunable) {
     if (rows == 2) return run(scan_kernel<T, Op, 2>);
     if (rows == 4) return run(scan_kernel<T, Op, 4>);

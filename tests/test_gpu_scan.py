@@ -96,6 +96,55 @@ def test_byte_scan_kernels(gpu, q, scratch, cfg):
     assert scratch.watchdog_trips() == 0
 
 
+# 8-bit plus: 8-row tiles on packed 16-bit pairs (element j and j + 8 in
+# one register), at several group sizes, with partial last tiles; 4 rows and
+# the work-group look-back take the generic kernel.
+@pytest.mark.parametrize("cfg", [None, (1024, 8), (512, 8), (256, 8), (64, 8), (1024, 4), (256, 0, 0, 1)])
+def test_packed_byte_plus_scans(gpu, q, scratch, cfg):
+    torch, _ = gpu
+    for dtype in (np.int8, np.uint8):
+        for n in (1, 15, 16, 17, 1000, 131072, 3 * 131072 + 77, (1 << 22) + 5):
+            rng = np.random.default_rng(n)
+            x = rng.integers(np.iinfo(dtype).min, np.iinfo(dtype).max + 1, n).astype(dtype)
+            init = dtype(-3 if dtype == np.int8 else 250)
+            got = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive", config=cfg)
+            assert np.array_equal(got, O.inclusive_scan(x, acpp.PLUS)), (dtype, n, cfg, "incl")
+            got = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive", init, config=cfg)
+            assert np.array_equal(got, O.inclusive_scan(x, acpp.PLUS, init)), (dtype, n, cfg, "incl+init")
+            got = _scan_case(torch, q, scratch, x, acpp.PLUS, "exclusive", init, config=cfg)
+            assert np.array_equal(got, O.exclusive_scan(x, init, acpp.PLUS)), (dtype, n, cfg, "excl")
+    assert scratch.watchdog_trips() == 0
+
+
+def test_packed_byte_plus_scan_2e28(gpu, q, scratch):
+    # full size, checked against the int64 running sum modulo 256 (exact)
+    torch, _ = gpu
+    n = 1 << 28
+    x = torch.randint(-128, 128, (n,), dtype=torch.int8, device="cuda")
+    y = torch.empty_like(x)
+    acpp.inclusive_scan(q, scratch, x, y)
+    q.wait()
+    want = (torch.cumsum(x.to(torch.int64), 0) & 0xff).to(torch.uint8).view(torch.int8)
+    assert torch.equal(y, want)
+    del want
+    acpp.exclusive_scan(q, scratch, x, y, np.int8(7))
+    q.wait()
+    want = ((torch.cumsum(x.to(torch.int64), 0) - x.to(torch.int64) + 7) & 0xff).to(torch.uint8).view(torch.int8)
+    assert torch.equal(y, want)
+    assert scratch.watchdog_trips() == 0
+
+
+def test_byte_scan_rows_rejected(gpu, q, scratch):
+    # only plus has the packed 8-row kernel; other operators take 4 rows
+    torch, _ = gpu
+    x = torch.zeros(1000, dtype=torch.int8, device="cuda")
+    y = torch.empty_like(x)
+    with pytest.raises(acpp.AcppError):
+        acpp.inclusive_scan(q, scratch, x, y, acpp.MAX, config=(256, 8))
+    with pytest.raises(acpp.AcppError):
+        acpp.inclusive_scan(q, scratch, x, y, acpp.PLUS, config=(256, 8, 0, 1))
+
+
 @pytest.mark.parametrize("offset", [1, 2, 3])
 def test_scan_misaligned_and_in_place(gpu, q, scratch, offset):
     torch, _ = gpu
