@@ -117,7 +117,10 @@ typedef struct acpp_launch_config {
   uint32_t work_group_size;  /* 64..1024, multiple of 64 */
   uint32_t items_per_thread; /* reduce: 16-byte loads in flight per thread
                                 (2, 4, 8, or 16 except for dot products);
-                                scans: 16-byte rows per thread (2, 4 or 8) */
+                                scans: 16-byte rows per thread (2, 4 or 8
+                                for 4- and 8-byte plus; 8 otherwise, except
+                                1-byte elements: 8 or 4 for plus, 4 for the
+                                other operators) */
   uint32_t groups_per_cu;    /* reduce and segmented: resident work-groups
                                 per CU (cap on the grid) */
   uint32_t flags;            /* variant bits, 0 = default:
