@@ -528,20 +528,24 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
 }
 
 // --------------------------------------------------------------------------
-// 8-bit plus scans on packed 16-bit pairs.
+// 8-bit scans on packed 16-bit pairs (plus, multiply, minimum, maximum and
+// the bitwise operators).
 //
 // The generic kernel widens every 8-bit element to a 32-bit work value, so
-// a 16-element vector takes 16 registers and only 4-row tiles fit.  For
-// plus, the 8-bit result is the sum modulo 256, so any wider modular sum
-// will do: register j of a row holds element j in its low and element
-// j + 8 in its high 16 bits (one v_perm_b32 each), and the row's
-// inclusive scan runs on both halves at once with packed 16-bit adds
-// (v_pk_add_u16), the high half then offset by the low half's total.  A row
-// takes 8 registers, so 8-row (128 KiB) tiles fit, with about half the
-// instructions per element.  Row totals, tile aggregates and prefixes are
-// 32-bit sums (exact modulo 2^32, hence modulo 256), and the output keeps
-// each sum's low byte.  2^28 int8: 109-111 us against 125-133 us for the
-// generic 4-row kernel (MI355X, same runs).
+// a 16-element vector takes 16 registers and only 4-row tiles fit.  Here
+// register j of a row holds element j in its low and element j + 8 in its
+// high 16 bits (one v_perm_b32 each), zero-extended, and the row's
+// inclusive scan runs on both halves at once with packed 16-bit operations
+// (v_pk_add_u16, v_pk_mul_lo_u16, v_pk_min_u16, ...), the high half then
+// combined with the low half's total.  A row takes 8 registers, so 8-row
+// (128 KiB) tiles fit, with about half the instructions per element.
+// Every result's low byte is the 8-bit result: sums and products are exact
+// modulo 2^16 (then modulo 2^32 across lanes and tiles), hence modulo 256;
+// the bitwise operators never leave the low byte; minimum and maximum of
+// signed bytes run on the bytes with their top bit flipped (x ^ 0x80),
+// which orders them as unsigned values, and the output is flipped back.
+// 2^28 int8 plus: 109-111 us against 125-133 us for the generic 4-row
+// kernel (MI355X, same runs).
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
@@ -549,11 +553,53 @@ __device__ __forceinline__ uint32_t as_u32(u16x2 x) { return __builtin_bit_cast(
 
 constexpr int packed8_rows = 8;
 
-template <bool FULL, class T, int R>
+// Whether the packed kernel covers operator Op on 8-bit element type T.
+template <class T, class Op>
+constexpr bool packed8_covers() {
+  return sizeof(T) == 1 && Op::id >= ops::op_plus && Op::id <= ops::op_bit_xor;
+}
+
+// The operator on packed pairs and on the 32-bit row totals; `ident` is
+// the identity byte (also what elements past the end read as), in the
+// flipped domain for signed minimum / maximum.
+template <class T, class Op>
+struct packed8_op {
+  static constexpr int id = Op::id;
+  static constexpr bool flip = (id == ops::op_min || id == ops::op_max) && std::is_signed_v<T>;
+  using Op32 = std::conditional_t<
+      id == ops::op_plus, ops::plus<uint32_t, uint32_t>,
+      std::conditional_t<
+          id == ops::op_multiply, ops::multiply<uint32_t, uint32_t>,
+          std::conditional_t<
+              id == ops::op_min, ops::minimum<uint32_t, uint32_t>,
+              std::conditional_t<id == ops::op_max, ops::maximum<uint32_t, uint32_t>,
+                                 std::conditional_t<id == ops::op_bit_and, ops::bit_and<uint32_t, uint32_t>,
+                                                    std::conditional_t<id == ops::op_bit_or,
+                                                                       ops::bit_or<uint32_t, uint32_t>,
+                                                                       ops::bit_xor<uint32_t, uint32_t>>>>>>>;
+  static constexpr uint32_t ident = (id == ops::op_multiply) ? 1u
+                                    : (id == ops::op_min || id == ops::op_bit_and) ? 0xffu
+                                                                                     : 0u;
+  __device__ __forceinline__ static u16x2 apply(u16x2 a, u16x2 b) {
+    if constexpr (id == ops::op_plus) return a + b;
+    else if constexpr (id == ops::op_multiply) return a * b;
+    else if constexpr (id == ops::op_min) return __builtin_elementwise_min(a, b);
+    else if constexpr (id == ops::op_max) return __builtin_elementwise_max(a, b);
+    else if constexpr (id == ops::op_bit_and) return a & b;
+    else if constexpr (id == ops::op_bit_or) return a | b;
+    else return a ^ b;
+  }
+};
+
+template <bool FULL, class T, class Op, int R>
 __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, uint32_t *s_wave, uint32_t &s_excl) {
-  static_assert(sizeof(T) == 1, "packed 8-bit scan");
-  using Op = ops::plus<uint32_t, uint32_t>;
+  static_assert(packed8_covers<T, Op>(), "packed 8-bit scan");
+  using PO = packed8_op<T, Op>;
+  using Op32 = typename PO::Op32;
   using SIO = status_io<uint32_t>;
+  constexpr uint32_t flip_mask = PO::flip ? 0x80808080u : 0u;
+  // elements past the end: the identity, stored before the flip
+  constexpr uint8_t fill = static_cast<uint8_t>(PO::ident ^ (PO::flip ? 0x80u : 0u));
   const int tid = threadIdx.x;
   const int B = blockDim.x;
   const int nw = B >> 6;
@@ -573,23 +619,25 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
 #pragma unroll
     for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
   } else {
-    // Elements past the end read as 0, the identity of plus.
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      uint32_t d[4] = {0u, 0u, 0u, 0u};
+      constexpr uint32_t fill4 = 0x01010101u * fill;
+      uint32_t d[4] = {fill4, fill4, fill4, fill4};
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         const uint32_t i = static_cast<uint32_t>(r * B + tid) * 16 + k;
-        if (i < in_range) d[k >> 2] |= static_cast<uint32_t>(in8[i]) << (8 * (k & 3));
+        if (i < in_range)
+          d[k >> 2] = (d[k >> 2] & ~(0xffu << (8 * (k & 3)))) | (static_cast<uint32_t>(in8[i]) << (8 * (k & 3)));
       }
       q[r] = u32x4{d[0], d[1], d[2], d[3]};
     }
   }
   u16x2 P[R][8];
   uint32_t wexcl[R];
+  const u16x2 id2 = {static_cast<unsigned short>(PO::ident), static_cast<unsigned short>(PO::ident)};
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint32_t d[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
+    const uint32_t d[4] = {q[r].x ^ flip_mask, q[r].y ^ flip_mask, q[r].z ^ flip_mask, q[r].w ^ flip_mask};
     // pair j = (byte j, byte j + 8), zero-extended to 16 bits each
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -597,31 +645,32 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
       P[r][j] = as_u16x2(__builtin_amdgcn_perm(d[(j >> 2) + 2], d[j >> 2], sel));
     }
 #pragma unroll
-    for (int j = 1; j < 8; ++j) P[r][j] = P[r][j - 1] + P[r][j];
+    for (int j = 1; j < 8; ++j) P[r][j] = PO::apply(P[r][j - 1], P[r][j]);
     // the high half continues after element 7
-    const u16x2 fix = {0, P[r][7].x};
+    const u16x2 fix = {id2.x, P[r][7].x};
 #pragma unroll
-    for (int j = 0; j < 8; ++j) P[r][j] = P[r][j] + fix;
-    const uint32_t incl = wave::inclusive_scan<Op>(static_cast<uint32_t>(P[r][7].y));
-    wexcl[r] = wave::shift_up<Op>(incl);
+    for (int j = 0; j < 8; ++j) P[r][j] = PO::apply(fix, P[r][j]);
+    const uint32_t incl = wave::inclusive_scan<Op32>(static_cast<uint32_t>(P[r][7].y));
+    wexcl[r] = wave::shift_up<Op32>(incl);
     if (lane == 63) s_wave[r * nw + w] = incl;
   }
   __syncthreads();
   if (w == 0) {
-    const uint32_t agg = tile_prefixes<Op, uint32_t>(R * nw, s_wave);
+    const uint32_t agg = tile_prefixes<Op32, uint32_t>(R * nw, s_wave);
     if (tile == 0) {
-      const uint32_t first = p.has_init ? static_cast<uint32_t>(p.init) : 0u;
+      const uint32_t first =
+          p.has_init ? ((static_cast<uint32_t>(p.init) & 0xffu) ^ (flip_mask & 0xffu)) : Op32::identity();
       if (lane == 0) {
-        if (p.num_tiles > 1) SIO::publish(p.status, 0, epoch, st_prefix, first + agg);
+        if (p.num_tiles > 1) SIO::publish(p.status, 0, epoch, st_prefix, Op32::apply(first, agg));
         s_excl = first;
       }
     } else {
       if (lane == 0) SIO::publish(p.status, tile, epoch, st_aggregate, agg);
       wait_own_stores();  // the aggregate is out before the first poll
-      const uint32_t excl = lookback<Op, uint32_t, lookback_slots>(p.status, tile, epoch, p.sync,
-                                                                   p.force_give_up != 0);
+      const uint32_t excl = lookback<Op32, uint32_t, lookback_slots>(p.status, tile, epoch, p.sync,
+                                                                     p.force_give_up != 0);
       if (lane == 0) {
-        if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, epoch, st_prefix, excl + agg);
+        if (tile + 1 < p.num_tiles) SIO::publish(p.status, tile, epoch, st_prefix, Op32::apply(excl, agg));
         s_excl = excl;
       }
     }
@@ -630,20 +679,20 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
   const uint32_t tile_prefix = s_excl;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint32_t P32 = tile_prefix + s_wave[r * nw + w] + wexcl[r];
+    const uint32_t P32 = Op32::apply(tile_prefix, Op32::apply(s_wave[r * nw + w], wexcl[r]));
     const unsigned short p16 = static_cast<unsigned short>(P32);
     const u16x2 pv = {p16, p16};
     u16x2 O[8];
     if (p.exclusive) {
       // element j takes the inclusive value of element j - 1; element 8
       // (high half of pair 0) that of element 7
-      const u16x2 e0 = {0, P[r][7].x};
-      O[0] = e0 + pv;
+      const u16x2 e0 = {id2.x, P[r][7].x};
+      O[0] = PO::apply(pv, e0);
 #pragma unroll
-      for (int j = 1; j < 8; ++j) O[j] = P[r][j - 1] + pv;
+      for (int j = 1; j < 8; ++j) O[j] = PO::apply(pv, P[r][j - 1]);
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) O[j] = P[r][j] + pv;
+      for (int j = 0; j < 8; ++j) O[j] = PO::apply(pv, P[r][j]);
     }
     // low bytes back in element order: [e0 e1 e8 e9] and [e2 e3 e10 e11]
     // give dwords 0 and 2; likewise pairs 4..7 give dwords 1 and 3
@@ -651,8 +700,10 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
     const uint32_t a1 = __builtin_amdgcn_perm(as_u32(O[3]), as_u32(O[2]), 0x06020400u);
     const uint32_t a2 = __builtin_amdgcn_perm(as_u32(O[5]), as_u32(O[4]), 0x06020400u);
     const uint32_t a3 = __builtin_amdgcn_perm(as_u32(O[7]), as_u32(O[6]), 0x06020400u);
-    const uint32_t o[4] = {__builtin_amdgcn_perm(a1, a0, 0x05040100u), __builtin_amdgcn_perm(a3, a2, 0x05040100u),
-                           __builtin_amdgcn_perm(a1, a0, 0x07060302u), __builtin_amdgcn_perm(a3, a2, 0x07060302u)};
+    const uint32_t o[4] = {__builtin_amdgcn_perm(a1, a0, 0x05040100u) ^ flip_mask,
+                           __builtin_amdgcn_perm(a3, a2, 0x05040100u) ^ flip_mask,
+                           __builtin_amdgcn_perm(a1, a0, 0x07060302u) ^ flip_mask,
+                           __builtin_amdgcn_perm(a3, a2, 0x07060302u) ^ flip_mask};
     if constexpr (FULL) {
       reinterpret_cast<u32x4 *>(out8)[r * B + tid] = u32x4{o[0], o[1], o[2], o[3]};
     } else {
@@ -665,7 +716,7 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
   }
 }
 
-template <class T>
+template <class T, class Op>
 __global__ __launch_bounds__(1024) void scan8_kernel(params<T> p) {
   constexpr int R = packed8_rows;
   __shared__ uint32_t s_tile;
@@ -680,9 +731,9 @@ __global__ __launch_bounds__(1024) void scan8_kernel(params<T> p) {
   const uint64_t tile_elems = static_cast<uint64_t>(blockDim.x) * R * 16;
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
   if (p.vec_io && base + tile_elems <= p.n)
-    scan8_tile<true, T, R>(p, tile, s_wave, s_excl);
+    scan8_tile<true, T, Op, R>(p, tile, s_wave, s_excl);
   else
-    scan8_tile<false, T, R>(p, tile, s_wave, s_excl);
+    scan8_tile<false, T, Op, R>(p, tile, s_wave, s_excl);
 }
 
 }  // namespace scan_detail

@@ -14,7 +14,8 @@ namespace scan_detail {
 // 153 us with 2 rows.  8-row tiles of 1-byte elements kept as raw bytes in
 // registers (and unpacked twice) measured slower than these 4-row tiles:
 // 145-151 us against 127-134 us at 2^28 int8 and uint8.  The 8-bit plus
-// scans take 8-row tiles on packed 16-bit pairs instead (scan8_kernel).
+// scans (operators other than the logical ones) take 8-row tiles on packed
+// 16-bit pairs instead (scan8_kernel).
 template <class T>
 constexpr int default_rows() {
   return sizeof(T) >= 2 ? 8 : 4;
@@ -39,10 +40,10 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     if (st != ACPP_SUCCESS) return st;
   }
 
-  // 8-bit plus scans run the packed-pair kernel (8 rows) unless the caller
-  // asks for 4 rows or the work-group-wide look-back, which only the
-  // generic kernel has.
-  constexpr bool packed8 = sizeof(T) == 1 && Op::id == ops::op_plus && !XF;
+  // 8-bit scans with operators other than the logical ones run the
+  // packed-pair kernel (8 rows) unless the caller asks for 4 rows or the
+  // work-group-wide look-back, which only the generic kernel has.
+  constexpr bool packed8 = packed8_covers<T, Op>() && !XF;
   const bool block_lb = cfg && (cfg->flags & variant_block_lookback);
   const int dflt_rows = (packed8 && !block_lb) ? packed8_rows : default_rows<T>();
 
@@ -133,7 +134,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     return s;
   };
   if constexpr (packed8) {
-    if (rows == packed8_rows) return run(scan8_kernel<T>);
+    if (rows == packed8_rows) return run(scan8_kernel<T, Op>);
   }
   if constexpr (tunable) {
     if (rows == 2) return run(scan_kernel<T, Op, 2>);

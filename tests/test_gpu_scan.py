@@ -96,23 +96,34 @@ def test_byte_scan_kernels(gpu, q, scratch, cfg):
     assert scratch.watchdog_trips() == 0
 
 
-# 8-bit plus: 8-row tiles on packed 16-bit pairs (element j and j + 8 in
-# one register), at several group sizes, with partial last tiles; 4 rows and
-# the work-group look-back take the generic kernel.
+# 8-bit scans on packed 16-bit pairs (element j and j + 8 in one register):
+# 8-row tiles at several group sizes, with partial last tiles, full-range
+# random bytes, every operator the packed kernel covers; 4 rows and the
+# work-group look-back take the generic kernel.
+PACKED_OPS = [acpp.PLUS, acpp.MULTIPLY, acpp.MIN, acpp.MAX, acpp.BIT_AND, acpp.BIT_OR, acpp.BIT_XOR]
+
+
 @pytest.mark.parametrize("cfg", [None, (1024, 8), (512, 8), (256, 8), (64, 8), (1024, 4), (256, 0, 0, 1)])
-def test_packed_byte_plus_scans(gpu, q, scratch, cfg):
+def test_packed_byte_scans(gpu, q, scratch, cfg):
     torch, _ = gpu
     for dtype in (np.int8, np.uint8):
-        for n in (1, 15, 16, 17, 1000, 131072, 3 * 131072 + 77, (1 << 22) + 5):
-            rng = np.random.default_rng(n)
-            x = rng.integers(np.iinfo(dtype).min, np.iinfo(dtype).max + 1, n).astype(dtype)
-            init = dtype(-3 if dtype == np.int8 else 250)
-            got = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive", config=cfg)
-            assert np.array_equal(got, O.inclusive_scan(x, acpp.PLUS)), (dtype, n, cfg, "incl")
-            got = _scan_case(torch, q, scratch, x, acpp.PLUS, "inclusive", init, config=cfg)
-            assert np.array_equal(got, O.inclusive_scan(x, acpp.PLUS, init)), (dtype, n, cfg, "incl+init")
-            got = _scan_case(torch, q, scratch, x, acpp.PLUS, "exclusive", init, config=cfg)
-            assert np.array_equal(got, O.exclusive_scan(x, init, acpp.PLUS)), (dtype, n, cfg, "excl")
+        for op in PACKED_OPS:
+            for n in (1, 15, 16, 17, 1000, 3 * 131072 + 77, (1 << 21) + 5):
+                rng = np.random.default_rng(n + 31 * op)
+                if op == acpp.MULTIPLY:
+                    x = rng.choice(np.array([1, 3, 5, 7, 255 if dtype == np.uint8 else -1], dtype=dtype), n)
+                elif op == acpp.BIT_AND:
+                    x = (np.iinfo(dtype).max if dtype == np.uint8 else -1) ^ (rng.random(n) < 0.001).astype(dtype)
+                    x = x.astype(dtype)
+                else:
+                    x = rng.integers(np.iinfo(dtype).min, np.iinfo(dtype).max + 1, n).astype(dtype)
+                init = dtype(-3 if dtype == np.int8 else 250)
+                got = _scan_case(torch, q, scratch, x, op, "inclusive", config=cfg)
+                assert np.array_equal(got, O.inclusive_scan(x, op)), (dtype, op, n, cfg, "incl")
+                got = _scan_case(torch, q, scratch, x, op, "inclusive", init, config=cfg)
+                assert np.array_equal(got, O.inclusive_scan(x, op, init)), (dtype, op, n, cfg, "incl+init")
+                got = _scan_case(torch, q, scratch, x, op, "exclusive", init, config=cfg)
+                assert np.array_equal(got, O.exclusive_scan(x, init, op)), (dtype, op, n, cfg, "excl")
     assert scratch.watchdog_trips() == 0
 
 
@@ -135,12 +146,13 @@ def test_packed_byte_plus_scan_2e28(gpu, q, scratch):
 
 
 def test_byte_scan_rows_rejected(gpu, q, scratch):
-    # only plus has the packed 8-row kernel; other operators take 4 rows
+    # the logical operators have no packed 8-row kernel and take 4 rows;
+    # the work-group look-back takes the generic kernel (4 rows)
     torch, _ = gpu
     x = torch.zeros(1000, dtype=torch.int8, device="cuda")
     y = torch.empty_like(x)
     with pytest.raises(acpp.AcppError):
-        acpp.inclusive_scan(q, scratch, x, y, acpp.MAX, config=(256, 8))
+        acpp.inclusive_scan(q, scratch, x, y, acpp.LOGICAL_AND, config=(256, 8))
     with pytest.raises(acpp.AcppError):
         acpp.inclusive_scan(q, scratch, x, y, acpp.PLUS, config=(256, 8, 0, 1))
 
