@@ -119,8 +119,8 @@ typedef struct acpp_launch_config {
                                 (2, 4, 8, or 16 except for dot products);
                                 scans: 16-byte rows per thread (2, 4 or 8
                                 for 4- and 8-byte plus; 8 otherwise, except
-                                1-byte elements: 8 or 4 for plus, 4 for the
-                                other operators) */
+                                1-byte elements: 8 or 4, and 4 for the
+                                logical operators) */
   uint32_t groups_per_cu;    /* reduce and segmented: resident work-groups
                                 per CU (cap on the grid) */
   uint32_t flags;            /* variant bits, 0 = default:

@@ -13,8 +13,8 @@ namespace scan_detail {
 // look-back chain, not the memory, then bounds the scan: 2^28 int8 took
 // 153 us with 2 rows.  8-row tiles of 1-byte elements kept as raw bytes in
 // registers (and unpacked twice) measured slower than these 4-row tiles:
-// 145-151 us against 127-134 us at 2^28 int8 and uint8.  The 8-bit plus
-// scans (operators other than the logical ones) take 8-row tiles on packed
+// 145-151 us against 127-134 us at 2^28 int8 and uint8.  The 8-bit scans
+// (operators other than the logical ones) take 8-row tiles on packed
 // 16-bit pairs instead (scan8_kernel).
 template <class T>
 constexpr int default_rows() {
