@@ -80,8 +80,13 @@ __global__ __launch_bounds__(1024) void histogram_kernel(const uint32_t *__restr
                                                          uint32_t copies_log2) {
   extern __shared__ uint32_t lds[];
   const uint32_t my_copy = threadIdx.x & ((1u << copies_log2) - 1u);
+  // Packed counters: bin b < half in the low 16 bits of word b, bin
+  // half + b in the high 16 bits, so the low (under skewed keys, hot) bins
+  // sit in consecutive words and banks; adjacent bins sharing a word put
+  // the two hottest Zipf bins on one address.
+  const uint32_t half = (num_bins + 1) / 2;
   if constexpr (!GLOBAL) {
-    const uint32_t words = PACKED ? (num_bins + 1) / 2 : (num_bins << copies_log2);
+    const uint32_t words = PACKED ? half : (num_bins << copies_log2);
     for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) lds[i] = 0;
     __syncthreads();
   }
@@ -91,11 +96,13 @@ __global__ __launch_bounds__(1024) void histogram_kernel(const uint32_t *__restr
     if constexpr (GLOBAL) {
       __hip_atomic_fetch_add(&bins[k], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if constexpr (PACKED) {
-      const uint32_t shift = (k & 1u) * 16u;
-      const uint32_t old = atomicAdd(&lds[k >> 1], 1u << shift);
+      const bool hi = k >= half;
+      const uint32_t word = hi ? k - half : k;
+      const uint32_t shift = hi ? 16u : 0u;
+      const uint32_t old = atomicAdd(&lds[word], 1u << shift);
       if (((old >> shift) & 0xffffu) == 0x7fffu) {
         // This work-item moved the field to 0x8000: drain it.
-        atomicAdd(&lds[k >> 1], 0u - (0x8000u << shift));
+        atomicAdd(&lds[word], 0u - (0x8000u << shift));
         __hip_atomic_fetch_add(&bins[k], 0x8000u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     } else {
@@ -135,7 +142,7 @@ __global__ __launch_bounds__(1024) void histogram_kernel(const uint32_t *__restr
     for (uint32_t b = threadIdx.x; b < num_bins; b += blockDim.x) {
       uint32_t v;
       if constexpr (PACKED) {
-        v = (lds[b >> 1] >> ((b & 1u) * 16u)) & 0xffffu;
+        v = (b >= half) ? (lds[b - half] >> 16) : (lds[b] & 0xffffu);
       } else {
         v = 0;
         for (uint32_t c = 0; c < (1u << copies_log2); ++c) v += lds[(b << copies_log2) + c];
