@@ -546,6 +546,10 @@ __global__ __launch_bounds__(1024) void scan_kernel(params<T> p) {
 // which orders them as unsigned values, and the output is flipped back.
 // 2^28 int8 plus: 109-111 us against 125-133 us for the generic 4-row
 // kernel (MI355X, same runs).
+// Transform scans (XF) apply the element transform to the raw bytes of a
+// row, four bytes per 32-bit operation (unary_detail::apply_bytes), before
+// the flip; a partial tile transforms its loaded bytes and only then fills
+// the bytes past the end with the identity.
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(u16x2, x); }
@@ -591,7 +595,7 @@ struct packed8_op {
   }
 };
 
-template <bool FULL, class T, class Op, int R>
+template <bool FULL, class T, class Op, int R, bool XF>
 __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, uint32_t *s_wave, uint32_t &s_excl) {
   static_assert(packed8_covers<T, Op>(), "packed 8-bit scan");
   using PO = packed8_op<T, Op>;
@@ -617,19 +621,43 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
   if constexpr (FULL) {
     const u32x4 *src = reinterpret_cast<const u32x4 *>(in8);
 #pragma unroll
-    for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+    for (int r = 0; r < R; ++r) {
+      q[r] = src[r * B + tid];
+      // With a transform, the loads are otherwise issued in any order ahead
+      // of the transform's branch, and the first row waits for nearly all.
+      if constexpr (XF) __builtin_amdgcn_sched_barrier(0);
+    }
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       constexpr uint32_t fill4 = 0x01010101u * fill;
-      uint32_t d[4] = {fill4, fill4, fill4, fill4};
+      if constexpr (XF) {
+        // transform the loaded bytes, then fill the bytes past the end
+        uint32_t d[4] = {0u, 0u, 0u, 0u};
+        const uint32_t first = static_cast<uint32_t>(r * B + tid) * 16;
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const uint32_t i = static_cast<uint32_t>(r * B + tid) * 16 + k;
-        if (i < in_range)
-          d[k >> 2] = (d[k >> 2] & ~(0xffu << (8 * (k & 3)))) | (static_cast<uint32_t>(in8[i]) << (8 * (k & 3)));
+        for (int k = 0; k < 16; ++k) {
+          if (first + k < in_range) d[k >> 2] |= static_cast<uint32_t>(in8[first + k]) << (8 * (k & 3));
+        }
+        unary_detail::apply_bytes<T>(p.unary, d);
+        const uint32_t cnt = in_range > first ? min(in_range - first, 16u) : 0u;
+#pragma unroll
+        for (int k4 = 0; k4 < 4; ++k4) {
+          const uint32_t vb = cnt > 4u * k4 ? min(cnt - 4u * k4, 4u) : 0u;  // loaded bytes in word k4
+          const uint32_t m = vb >= 4u ? ~0u : ((1u << (8u * vb)) - 1u);
+          d[k4] = (d[k4] & m) | (fill4 & ~m);
+        }
+        q[r] = u32x4{d[0], d[1], d[2], d[3]};
+      } else {
+        uint32_t d[4] = {fill4, fill4, fill4, fill4};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const uint32_t i = static_cast<uint32_t>(r * B + tid) * 16 + k;
+          if (i < in_range)
+            d[k >> 2] = (d[k >> 2] & ~(0xffu << (8 * (k & 3)))) | (static_cast<uint32_t>(in8[i]) << (8 * (k & 3)));
+        }
+        q[r] = u32x4{d[0], d[1], d[2], d[3]};
       }
-      q[r] = u32x4{d[0], d[1], d[2], d[3]};
     }
   }
   u16x2 P[R][8];
@@ -637,7 +665,10 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
   const u16x2 id2 = {static_cast<unsigned short>(PO::ident), static_cast<unsigned short>(PO::ident)};
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint32_t d[4] = {q[r].x ^ flip_mask, q[r].y ^ flip_mask, q[r].z ^ flip_mask, q[r].w ^ flip_mask};
+    uint32_t d[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
+    if constexpr (XF && FULL) unary_detail::apply_bytes<T>(p.unary, d);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] ^= flip_mask;
     // pair j = (byte j, byte j + 8), zero-extended to 16 bits each
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -716,7 +747,7 @@ __device__ __forceinline__ void scan8_tile(const params<T> &p, uint32_t tile, ui
   }
 }
 
-template <class T, class Op>
+template <class T, class Op, bool XF>
 __global__ __launch_bounds__(1024) void scan8_kernel(params<T> p) {
   constexpr int R = packed8_rows;
   __shared__ uint32_t s_tile;
@@ -731,9 +762,9 @@ __global__ __launch_bounds__(1024) void scan8_kernel(params<T> p) {
   const uint64_t tile_elems = static_cast<uint64_t>(blockDim.x) * R * 16;
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
   if (p.vec_io && base + tile_elems <= p.n)
-    scan8_tile<true, T, Op, R>(p, tile, s_wave, s_excl);
+    scan8_tile<true, T, Op, R, XF>(p, tile, s_wave, s_excl);
   else
-    scan8_tile<false, T, Op, R>(p, tile, s_wave, s_excl);
+    scan8_tile<false, T, Op, R, XF>(p, tile, s_wave, s_excl);
 }
 
 }  // namespace scan_detail

@@ -40,10 +40,10 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     if (st != ACPP_SUCCESS) return st;
   }
 
-  // 8-bit scans with operators other than the logical ones run the
-  // packed-pair kernel (8 rows) unless the caller asks for 4 rows or the
-  // work-group-wide look-back, which only the generic kernel has.
-  constexpr bool packed8 = packed8_covers<T, Op>() && !XF;
+  // 8-bit scans and transform scans with operators other than the logical
+  // ones run the packed-pair kernel (8 rows) unless the caller asks for 4
+  // rows or the work-group-wide look-back, which only the generic kernel has.
+  constexpr bool packed8 = packed8_covers<T, Op>();
   const bool block_lb = cfg && (cfg->flags & variant_block_lookback);
   const int dflt_rows = (packed8 && !block_lb) ? packed8_rows : default_rows<T>();
 
@@ -134,7 +134,7 @@ acpp_status launch(acpp_allocation_group *g, hipStream_t stream, const T *in, ui
     return s;
   };
   if constexpr (packed8) {
-    if (rows == packed8_rows) return run(scan8_kernel<T, Op>);
+    if (rows == packed8_rows) return run(scan8_kernel<T, Op, XF>);
   }
   if constexpr (tunable) {
     if (rows == 2) return run(scan_kernel<T, Op, 2>);

@@ -114,6 +114,66 @@ __device__ __forceinline__ W apply_one(int code, W w) {
   return v[0][0];
 }
 
+// Transform C of the four 1-byte elements (type T) packed in x, byte k of
+// the result being apply<C, T> of byte k.  Plain 32-bit arithmetic whose
+// carries and borrows never cross a byte (H: the top bit of every byte):
+// negate computes (0x80 | 0) - (b & 0x7f) per byte and restores the top
+// bit, is_zero / is_nonzero read the top bit of (b & 0x7f) + 0x7f | b, and
+// square multiplies the even and the odd bytes as 16-bit halves.
+template <int C, class T>
+__device__ __forceinline__ uint32_t apply_bytes4(uint32_t x) {
+  static_assert(sizeof(T) == 1 && std::is_integral_v<T>, "1-byte integer elements");
+  typedef unsigned short half2 __attribute__((ext_vector_type(2)));
+  constexpr uint32_t H = 0x80808080u;
+  constexpr uint32_t L = 0x7f7f7f7fu;
+  if constexpr (C == u_negate) {
+    return (H - (x & L)) ^ (~x & H);
+  } else if constexpr (C == u_abs) {
+    if constexpr (std::is_signed_v<T>) {
+      const uint32_t neg = (H - (x & L)) ^ (~x & H);
+      const uint32_t m = ((x & H) >> 7) * 0xffu;  // 0xff in every negative byte
+      return (neg & m) | (x & ~m);
+    } else {
+      return x;
+    }
+  } else if constexpr (C == u_square) {
+    const half2 e = __builtin_bit_cast(half2, x & 0x00ff00ffu);
+    const half2 o = __builtin_bit_cast(half2, (x >> 8) & 0x00ff00ffu);
+    return (__builtin_bit_cast(uint32_t, e * e) & 0x00ff00ffu) |
+           ((__builtin_bit_cast(uint32_t, o * o) & 0x00ff00ffu) << 8);
+  } else if constexpr (C == u_twice) {
+    return (x << 1) & 0xfefefefeu;
+  } else if constexpr (C == u_bit_not) {
+    return ~x;
+  } else if constexpr (C == u_is_zero || C == u_is_nonzero) {
+    const uint32_t y = ((x & L) + L) | x;  // top bit of a byte set: byte != 0
+    return C == u_is_nonzero ? (y & H) >> 7 : (~y & H) >> 7;
+  } else {
+    return x;
+  }
+}
+
+template <int C, class T>
+__device__ __forceinline__ void each_bytes(uint32_t (&d)[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d[k] = apply_bytes4<C, T>(d[k]);
+}
+
+// Apply transform `code` to 16 packed 1-byte elements (one uniform branch).
+template <class T>
+__device__ __forceinline__ void apply_bytes(int code, uint32_t (&d)[4]) {
+  switch (code) {
+    case u_negate: each_bytes<u_negate, T>(d); break;
+    case u_abs: each_bytes<u_abs, T>(d); break;
+    case u_square: each_bytes<u_square, T>(d); break;
+    case u_twice: each_bytes<u_twice, T>(d); break;
+    case u_bit_not: each_bytes<u_bit_not, T>(d); break;
+    case u_is_zero: each_bytes<u_is_zero, T>(d); break;
+    case u_is_nonzero: each_bytes<u_is_nonzero, T>(d); break;
+    default: break;
+  }
+}
+
 // Host-side validation: whether `code` is a transform of element type T.
 template <class T>
 inline bool valid_code(int code) {

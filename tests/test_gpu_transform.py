@@ -42,13 +42,13 @@ def _reduce(torch, q, scratch, x, init, op, code, config=None):
     return host(out)[0]
 
 
-def _scan(torch, q, scratch, x, op, code, kind, init=None):
+def _scan(torch, q, scratch, x, op, code, kind, init=None, config=None):
     xd = dev(torch, x)
     yd = torch.empty_like(xd)
     if kind == "inclusive":
-        acpp.transform_inclusive_scan(q, scratch, xd, yd, op, code, init)
+        acpp.transform_inclusive_scan(q, scratch, xd, yd, op, code, init, config=config)
     else:
-        acpp.transform_exclusive_scan(q, scratch, xd, yd, init, op, code)
+        acpp.transform_exclusive_scan(q, scratch, xd, yd, init, op, code, config=config)
     q.wait()
     return host(yd)
 
@@ -107,6 +107,41 @@ def test_transform_scans_int_exact(gpu, q, scratch, dtype):
                 got = _scan(torch, q, scratch, x, op, code, "exclusive", init)
                 assert np.array_equal(got, O.transform_exclusive_scan(x, init, op, code)), \
                     (dtype, op, n, code, "excl")
+
+
+PACKED_OPS = [acpp.PLUS, acpp.MULTIPLY, acpp.MIN, acpp.MAX, acpp.BIT_AND, acpp.BIT_OR, acpp.BIT_XOR]
+
+
+# 8-bit transform scans run the packed-pair kernel (8 rows: default, 1024- and
+# 256-wide) with the transform applied to four bytes per 32-bit operation,
+# or the generic kernel (4 rows).  Every byte value occurs at every position
+# of a 16-byte vector; the sizes give one partial tile, three full 1024 x 8
+# tiles plus a ragged tail, and twelve full 256 x 8 tiles plus a tail.
+@pytest.mark.parametrize("cfg", [None, (1024, 8), (256, 8), (1024, 4)])
+def test_packed_byte_transform_scans(gpu, q, scratch, cfg):
+    torch, _ = gpu
+    for dtype in (np.int8, np.uint8):
+        info = np.iinfo(dtype)
+        for op in PACKED_OPS:
+            for n in (1, 17, 1000, 3 * 131072 + 77):
+                rng = np.random.default_rng(n + 31 * op)
+                if op == acpp.MULTIPLY:
+                    x = rng.choice(np.array([1, 3, 5, 7, 255 if dtype == np.uint8 else -1], dtype=dtype), n)
+                else:
+                    x = rng.integers(info.min, info.max + 1, n).astype(dtype)
+                    if n >= 4096:
+                        x[:4096] = np.repeat(np.arange(256), 16).astype(np.uint8).view(dtype)
+                init = dtype(-3 if dtype == np.int8 else 250)
+                for code in INT_CODES:
+                    got = _scan(torch, q, scratch, x, op, code, "inclusive", config=cfg)
+                    assert np.array_equal(got, O.transform_inclusive_scan(x, op, code)), (dtype, op, n, code, cfg)
+                    got = _scan(torch, q, scratch, x, op, code, "inclusive", init, config=cfg)
+                    assert np.array_equal(got, O.transform_inclusive_scan(x, op, code, init)), \
+                        (dtype, op, n, code, cfg, "init")
+                    got = _scan(torch, q, scratch, x, op, code, "exclusive", init, config=cfg)
+                    assert np.array_equal(got, O.transform_exclusive_scan(x, init, op, code)), \
+                        (dtype, op, n, code, cfg, "excl")
+    assert scratch.watchdog_trips() == 0
 
 
 @pytest.mark.parametrize("dtype", FP_TYPES)
