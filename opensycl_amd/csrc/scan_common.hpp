@@ -375,6 +375,9 @@ __device__ __forceinline__ void scan_tile(const params<T> &p, uint32_t tile,
         q[r] = __builtin_nontemporal_load(src + r * B + tid);
       else
         q[r] = src[r * B + tid];
+      // With a transform, the loads are otherwise issued in any order ahead
+      // of the transform's branch, and the first row waits for nearly all.
+      if constexpr (XF) __builtin_amdgcn_sched_barrier(0);
     }
   } else {
 #pragma unroll
@@ -396,16 +399,16 @@ __device__ __forceinline__ void scan_tile(const params<T> &p, uint32_t tile,
 #pragma unroll
     for (int k = 0; k < VEC; ++k) v[r][k] = elem_traits<T>::to_work(tmp[k]);
   }
-  if constexpr (XF && VEC <= 4) unary_detail::apply_tile<T>(p.unary, v);
 
   // Per row: the work-item-local inclusive scan of the row vector, then the
   // wave scan of the per-lane row totals.
   W wexcl[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    // Narrow elements: one uniform switch per row (a whole tile of widened
-    // 8- or 16-bit elements under one switch spilled to scratch).
-    if constexpr (XF && VEC > 4) unary_detail::apply_row<T>(p.unary, v[r]);
+    // One uniform switch per row, so that each row is transformed as its
+    // load arrives (a whole tile of widened 8- or 16-bit elements under one
+    // switch also spilled to scratch).
+    if constexpr (XF) unary_detail::apply_row<T>(p.unary, v[r]);
     if constexpr (!FULL) {
       // Elements past the end act as the identity.
 #pragma unroll

@@ -12,9 +12,10 @@
 // 2^bits(T) (abs of the most negative value is itself), and the kernels then
 // combine the transformed values exactly as they combine plain elements.
 //
-// The code is uniform across a launch.  A kernel applies it to a whole
-// register tile with one scalar branch (apply_tile), so the transform costs
-// one uniform switch per loaded batch, not per element.
+// The code is uniform across a launch.  A kernel applies it to a register
+// tile or to one loaded row with one scalar branch (apply_tile, apply_row;
+// apply_bytes for 16 packed 1-byte elements), so the transform costs one
+// uniform switch per loaded batch, not per element.
 #pragma once
 
 #include "internal.hpp"
