@@ -257,6 +257,12 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
   using W = typename elem_traits<T>::work;
   using V = typename M::value;
   constexpr int VEC = 16 / sizeof(T);
+  // Sums of 1-byte integers: the result keeps only the low byte, so the sum
+  // of the bytes taken as unsigned values equals the sum of the elements
+  // modulo 256.  v_sad_u8 against zero adds four bytes to the accumulator
+  // in one instruction (a transform first acts on the packed bytes).
+  constexpr bool BYTE_SUM = sizeof(T) == 1 && sizeof(OutT) == 1 && std::is_integral_v<T> &&
+                            Op::id == ops::op_plus && !DOT && std::is_same_v<V, W> && sizeof(W) == 4;
 
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t gtid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -276,8 +282,21 @@ __global__ __launch_bounds__(1024) void reduce_kernel(params<T, M, OutT> p) {
     for (int u = 0; u < UNROLL; ++u) {
       qa[u] = load16<NT>(va, c + u * stride);
       if constexpr (DOT) qb[u] = load16_a4<NT>(vb, c + u * stride);
+      // keeps the loads in order ahead of the transform's branch
+      if constexpr (BYTE_SUM && XF) __builtin_amdgcn_sched_barrier(0);
     }
-    if constexpr (XF && VEC > 4) {
+    if constexpr (BYTE_SUM) {
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        uint32_t d[4] = {qa[u].x, qa[u].y, qa[u].z, qa[u].w};
+        if constexpr (XF) unary_detail::apply_bytes<T>(p.unary, d);
+        uint32_t a = static_cast<uint32_t>(acc[u]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a = __builtin_amdgcn_sad_u8(d[k], 0u, a);
+        acc[u] = static_cast<V>(a);
+      }
+      continue;
+    } else if constexpr (XF && VEC > 4) {
       // Narrow elements (8 or 16 per load): one uniform switch per row.  A
       // whole batch of 8-bit elements widened at once (64 values) spilled.
 #pragma unroll

@@ -75,6 +75,33 @@ def test_transform_reduce_int_exact(gpu, q, scratch, dtype):
                 assert got == O.transform_reduce(x, init, op, code), (dtype, op, n, code)
 
 
+# 1-byte sums large enough for the grid-stride main loop (2^25 + 12345
+# elements), where the bytes are summed four at a time: the plain sum under
+# several launch shapes (the last with non-temporal loads, flag 2) and from
+# misaligned starts (scalar head), and every transform, against the oracle.
+# Every byte value occurs at every position of a 16-byte vector.
+def test_byte_sums_main_loop(gpu, q, scratch):
+    torch, _ = gpu
+    for dtype in (np.int8, np.uint8):
+        info = np.iinfo(dtype)
+        rng = np.random.default_rng(5)
+        x = rng.integers(info.min, info.max + 1, (1 << 25) + 12345).astype(dtype)
+        x[:4096] = np.repeat(np.arange(256), 16).astype(np.uint8).view(dtype)
+        init = dtype(-7 if dtype == np.int8 else 200)
+        for code in INT_CODES:
+            assert _reduce(torch, q, scratch, x, init, acpp.PLUS, code) == \
+                O.transform_reduce(x, init, acpp.PLUS, code), (dtype, code)
+        xd = dev(torch, x)
+        out = torch.zeros(1, dtype=xd.dtype, device="cuda")
+        for off in (0, 1, 5):
+            want = O.reduce(x[off:], init, acpp.PLUS)
+            for cfg in (None, (512, 2), (256, 8), (1024, 16), (1024, 4, 0, 2)):
+                acpp.reduce(q, scratch, xd[off:], out, init, acpp.PLUS, config=cfg)
+                q.wait()
+                assert host(out)[0] == want, (dtype, off, cfg)
+    assert scratch.watchdog_trips() == 0
+
+
 @pytest.mark.parametrize("dtype", FP_TYPES)
 def test_transform_reduce_fp(gpu, q, scratch, dtype):
     torch, _ = gpu
