@@ -44,6 +44,10 @@ struct copy_if_abi {
                               nullptr);                                                            \
     }                                                                                              \
   };
+ACPP_FACADE_COPY_IF(signed char, i8, int8_t)
+ACPP_FACADE_COPY_IF(unsigned char, u8, uint8_t)
+ACPP_FACADE_COPY_IF(short, i16, int16_t)
+ACPP_FACADE_COPY_IF(unsigned short, u16, uint16_t)
 ACPP_FACADE_COPY_IF(int, i32, int32_t)
 ACPP_FACADE_COPY_IF(unsigned int, u32, uint32_t)
 ACPP_FACADE_COPY_IF(long, i64, int64_t)
@@ -64,8 +68,8 @@ event copy_if(queue &q, util::allocation_group &scratch, ForwardIt1 first, Forwa
               const std::vector<event> &deps = {}) {
   using T = typename detail::contiguous<ForwardIt1>::value;
   static_assert(detail::copy_if_abi<T>::supported,
-                "acpp::algorithms::copy_if: element type must be int32, uint32, int64, uint64, float "
-                "or double");
+                "acpp::algorithms::copy_if: element type must be an 8-, 16-, 32- or 64-bit integer, "
+                "float or double");
   static_assert(std::is_same<T, typename detail::contiguous<ForwardIt2>::value>::value,
                 "acpp::algorithms::copy_if: input and output value types must be equal");
   using P = predicate_of<std::decay_t<UnaryPredicate>, T>;

@@ -537,6 +537,10 @@ acpp_status acpp_reduction_multi_identity_f64(acpp_allocation_group *scratch, ac
 #define ACPP_PRED_GE 6      /* x >= operand */
 #define ACPP_PRED_MASK_EQ 7 /* (x & mask) == operand, integers only */
 #define ACPP_PRED_STENCIL 8 /* stencil[i] != 0 (the _stencil entry points) */
+acpp_status acpp_copy_if_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, int8_t *d_first, int pred, int8_t operand, int8_t mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, uint8_t *d_first, int pred, uint8_t operand, uint8_t mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, int16_t *d_first, int pred, int16_t operand, int16_t mask, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, uint16_t *d_first, int pred, uint16_t operand, uint16_t mask, uint64_t *num_copied, const acpp_launch_config *config);
 acpp_status acpp_copy_if_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, int32_t *d_first, int pred, int32_t operand, int32_t mask, uint64_t *num_copied, const acpp_launch_config *config);
 acpp_status acpp_copy_if_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, uint32_t *d_first, int pred, uint32_t operand, uint32_t mask, uint64_t *num_copied, const acpp_launch_config *config);
 acpp_status acpp_copy_if_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, int64_t *d_first, int pred, int64_t operand, int64_t mask, uint64_t *num_copied, const acpp_launch_config *config);
@@ -544,6 +548,10 @@ acpp_status acpp_copy_if_u64(acpp_allocation_group *scratch, acpp_stream_t strea
 acpp_status acpp_copy_if_f32(acpp_allocation_group *scratch, acpp_stream_t stream, const float *first, uint64_t n, float *d_first, int pred, float operand, float mask, uint64_t *num_copied, const acpp_launch_config *config);
 acpp_status acpp_copy_if_f64(acpp_allocation_group *scratch, acpp_stream_t stream, const double *first, uint64_t n, double *d_first, int pred, double operand, double mask, uint64_t *num_copied, const acpp_launch_config *config);
 /* Predicate evaluated by the caller: copies x[i] where stencil[i] != 0. */
+acpp_status acpp_copy_if_stencil_i8(acpp_allocation_group *scratch, acpp_stream_t stream, const int8_t *first, uint64_t n, const uint8_t *stencil, int8_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_u8(acpp_allocation_group *scratch, acpp_stream_t stream, const uint8_t *first, uint64_t n, const uint8_t *stencil, uint8_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_i16(acpp_allocation_group *scratch, acpp_stream_t stream, const int16_t *first, uint64_t n, const uint8_t *stencil, int16_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
+acpp_status acpp_copy_if_stencil_u16(acpp_allocation_group *scratch, acpp_stream_t stream, const uint16_t *first, uint64_t n, const uint8_t *stencil, uint16_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
 acpp_status acpp_copy_if_stencil_i32(acpp_allocation_group *scratch, acpp_stream_t stream, const int32_t *first, uint64_t n, const uint8_t *stencil, int32_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
 acpp_status acpp_copy_if_stencil_u32(acpp_allocation_group *scratch, acpp_stream_t stream, const uint32_t *first, uint64_t n, const uint8_t *stencil, uint32_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);
 acpp_status acpp_copy_if_stencil_i64(acpp_allocation_group *scratch, acpp_stream_t stream, const int64_t *first, uint64_t n, const uint8_t *stencil, int64_t *d_first, uint64_t *num_copied, const acpp_launch_config *config);

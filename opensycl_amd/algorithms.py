@@ -503,7 +503,7 @@ def atomic_fetch_strided(q: Queue, scratch: AllocationGroup, op: str, values, ta
 PRED_NONZERO, PRED_EQ, PRED_NE, PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_MASK_EQ = range(8)
 
 
-_COPY_IF_TAGS = ("i32", "u32", "i64", "u64", "f32", "f64")
+_COPY_IF_TAGS = ("i8", "u8", "i16", "u16", "i32", "u32", "i64", "u64", "f32", "f64")
 
 
 def copy_if(q: Queue, scratch: AllocationGroup, x, d, pred: int, operand=0, mask=0, num_copied=None,
@@ -517,7 +517,7 @@ def copy_if(q: Queue, scratch: AllocationGroup, x, d, pred: int, operand=0, mask
         _check_out(num_copied, _torch().uint64, 1, "copy_if num_copied")
     tag = _tag(x.dtype)
     if tag not in _COPY_IF_TAGS:
-        raise TypeError(f"copy_if takes 4- or 8-byte elements (i32, u32, i64, u64, f32, f64), not {tag}")
+        raise TypeError(f"copy_if takes integer, f32 or f64 elements, not {tag}")
     fn = getattr(_abi.load(), f"acpp_copy_if_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(d), int(pred), _scalar(tag, operand),
              _scalar(tag, mask), _ptr(num_copied) if num_copied is not None else None, _cfg(config)))
@@ -535,7 +535,7 @@ def copy_if_stencil(q: Queue, scratch: AllocationGroup, x, stencil, d, num_copie
         _check_out(num_copied, _torch().uint64, 1, "copy_if_stencil num_copied")
     tag = _tag(x.dtype)
     if tag not in _COPY_IF_TAGS:
-        raise TypeError(f"copy_if takes 4- or 8-byte elements (i32, u32, i64, u64, f32, f64), not {tag}")
+        raise TypeError(f"copy_if takes integer, f32 or f64 elements, not {tag}")
     fn = getattr(_abi.load(), f"acpp_copy_if_stencil_{tag}")
     check(fn(scratch.handle, q.handle, _ptr(x), x.numel(), _ptr(stencil), _ptr(d),
              _ptr(num_copied) if num_copied is not None else None, _cfg(config)))

@@ -102,6 +102,51 @@ __device__ __forceinline__ void select_bits(const params<T> &p, const T (&v)[R][
   }
 }
 
+// Element k of a 16-byte vector of T.
+template <class T>
+__device__ __forceinline__ T elem(const u32x4 &q, int k) {
+  T tmp[16 / sizeof(T)];
+  __builtin_memcpy(tmp, &q, 16);
+  return tmp[k];
+}
+
+// select_bits for 1- and 2-byte elements kept packed in their vectors (q);
+// in_range: the tile's elements before the end of the input.
+template <int PRED, class T, int R>
+__device__ __forceinline__ void select_bits_packed(const params<T> &p, const u32x4 (&q)[R], uint64_t base,
+                                                   uint32_t in_range, bool full, uint32_t (&fm)[R]) {
+  constexpr int VEC = 16 / sizeof(T);
+  const int B = blockDim.x;
+  if constexpr (PRED == ACPP_PRED_STENCIL) {
+    if (full && p.stencil_vec) {
+      // The VEC stencil bytes of a data vector in one load.
+      using SW = std::conditional_t<VEC == 16, u32x4, uint64_t>;
+      const SW *sw = reinterpret_cast<const SW *>(p.stencil + base);
+      SW bytes[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) bytes[r] = sw[r * B + threadIdx.x];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        uint32_t words[4] = {};
+        __builtin_memcpy(words, &bytes[r], sizeof(SW));
+        fm[r] = 0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) fm[r] |= (((words[k >> 2] >> (8 * (k & 3))) & 0xffu) != 0u ? 1u : 0u) << k;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    fm[r] = 0;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+      const uint32_t i = static_cast<uint32_t>(r * B + threadIdx.x) * VEC + k;
+      if ((full || i < in_range) && eval<PRED>(p, elem<T>(q[r], k), base + i)) fm[r] |= 1u << k;
+    }
+  }
+}
+
 // Shared words of a copy_if work-group.
 template <int R>
 struct tile_shared {
@@ -129,14 +174,38 @@ __device__ __forceinline__ void copy_if_tile(const params<T> &p, uint32_t tile, 
   const uint64_t tile_elems = static_cast<uint64_t>(B) * R * VEC;
   const uint64_t base = static_cast<uint64_t>(tile) * tile_elems;
 
-  T v[R][VEC];
+  // 1- and 2-byte elements stay packed in their 16-byte vectors (four
+  // registers a row) and are extracted where they are used: 8 rows of 8 or
+  // 16 unpacked elements spill.
+  constexpr bool NARROW = sizeof(T) < 4;
+  u32x4 q[R];
+  T v[NARROW ? 1 : R][VEC];
   if constexpr (FULL) {
     const scan_detail::u32x4_a4 *src = reinterpret_cast<const scan_detail::u32x4_a4 *>(p.in + base);
-    u32x4 q[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) q[r] = src[r * B + tid];
+    if constexpr (!NARROW) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) __builtin_memcpy(v[r], &q[r], 16);
+      for (int r = 0; r < R; ++r) __builtin_memcpy(v[r], &q[r], 16);
+    }
+  } else if constexpr (NARROW) {
+    const uint32_t in_range = static_cast<uint32_t>(p.n > base ? min(p.n - base, tile_elems) : 0);
+    const T *in_tile = p.in + base;
+    using U = std::make_unsigned_t<T>;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // each element ORed into its word as it arrives (a select per
+      // element kept all 128 loads of a row set live and spilled)
+      uint32_t d[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) {
+        const uint32_t i = static_cast<uint32_t>(r * B + tid) * VEC + k;
+        constexpr int sz = static_cast<int>(sizeof(T));
+        if (i < in_range)
+          d[(k * sz) >> 2] |= static_cast<uint32_t>(static_cast<U>(in_tile[i])) << (8 * ((k * sz) & 3));
+      }
+      q[r] = u32x4{d[0], d[1], d[2], d[3]};
+    }
   } else {
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -147,7 +216,21 @@ __device__ __forceinline__ void copy_if_tile(const params<T> &p, uint32_t tile, 
       }
   }
   uint32_t fm[R];
-  select_bits<PRED>(p, v, base, FULL, fm);
+  if constexpr (NARROW) {
+    const uint32_t in_range = FULL ? static_cast<uint32_t>(tile_elems)
+                                   : static_cast<uint32_t>(p.n > base ? min(p.n - base, tile_elems) : 0);
+    select_bits_packed<PRED>(p, q, base, in_range, FULL, fm);
+    // The compaction below extracts the elements again from the packed
+    // vectors instead of keeping the predicate's extracted copies alive.
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      uint32_t a = q[r].x, b = q[r].y, c = q[r].z, d = q[r].w;
+      asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+      q[r] = u32x4{a, b, c, d};
+    }
+  } else {
+    select_bits<PRED>(p, v, base, FULL, fm);
+  }
   // Ranks inside each wave row; the (row, wave) counts go to LDS.
   uint32_t wexcl[R];
 #pragma unroll
@@ -184,7 +267,10 @@ __device__ __forceinline__ void copy_if_tile(const params<T> &p, uint32_t tile, 
     const uint32_t bits = fm[r];
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
-      if (bits & (1u << k)) s_stage[pos++] = v[r][k];
+      if (bits & (1u << k)) {
+        if constexpr (NARROW) s_stage[pos++] = elem<T>(q[r], k);
+        else s_stage[pos++] = v[r][k];
+      }
     }
   }
   // Wave 0 looks back for the tile's output offset.
@@ -371,6 +457,10 @@ extern "C" {
                                     "acpp_copy_if_stencil_" #tag);                                         \
   }
 
+ACPP_DEFINE_COPY_IF(i8, int8_t)
+ACPP_DEFINE_COPY_IF(u8, uint8_t)
+ACPP_DEFINE_COPY_IF(i16, int16_t)
+ACPP_DEFINE_COPY_IF(u16, uint16_t)
 ACPP_DEFINE_COPY_IF(i32, int32_t)
 ACPP_DEFINE_COPY_IF(u32, uint32_t)
 ACPP_DEFINE_COPY_IF(i64, int64_t)

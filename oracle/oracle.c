@@ -647,12 +647,20 @@ int oracle_fetch_max_strided_i32(const int32_t *values, uint64_t n, int32_t *slo
   }
 
 #define COPY_IF_MASK(T, x, mask) COPY_IF_MASK_##T(x, mask)
+#define COPY_IF_MASK_int8_t(x, mask) ((int8_t)((x) & (mask)))
+#define COPY_IF_MASK_uint8_t(x, mask) ((uint8_t)((x) & (mask)))
+#define COPY_IF_MASK_int16_t(x, mask) ((int16_t)((x) & (mask)))
+#define COPY_IF_MASK_uint16_t(x, mask) ((uint16_t)((x) & (mask)))
 #define COPY_IF_MASK_int32_t(x, mask) ((int32_t)((x) & (mask)))
 #define COPY_IF_MASK_uint32_t(x, mask) ((uint32_t)((x) & (mask)))
 #define COPY_IF_MASK_int64_t(x, mask) ((int64_t)((x) & (mask)))
 #define COPY_IF_MASK_uint64_t(x, mask) ((uint64_t)((x) & (mask)))
 #define COPY_IF_MASK_float(x, mask) ((float)0)
 #define COPY_IF_MASK_double(x, mask) ((double)0)
+DEF_COPY_IF(i8, int8_t, 1)
+DEF_COPY_IF(u8, uint8_t, 1)
+DEF_COPY_IF(i16, int16_t, 1)
+DEF_COPY_IF(u16, uint16_t, 1)
 DEF_COPY_IF(i32, int32_t, 1)
 DEF_COPY_IF(u32, uint32_t, 1)
 DEF_COPY_IF(i64, int64_t, 1)

@@ -189,6 +189,11 @@ static void test_copy_if(acpp::queue &q, alg::util::allocation_cache &cache) {
     copy_if_case(q, g, fx, acpp::is_less_equal(-10.0), "x <= -10.0");
     std::vector<unsigned long> ux(x.begin(), x.end());
     copy_if_case(q, g, ux, acpp::is_not_equal_to(0ul), "unsigned long x != 0");
+    std::vector<signed char> bx(x.begin(), x.end());
+    copy_if_case(q, g, bx, acpp::is_less_than(static_cast<signed char>(-5)), "int8 x < -5");
+    std::vector<unsigned short> hx(x.begin(), x.end());
+    copy_if_case(q, g, hx, acpp::has_bits(static_cast<unsigned short>(3), static_cast<unsigned short>(1)),
+                 "uint16 (x & 3) == 1");
   }
   // empty range without a count: a complete event
   device_buffer<int> e(1);

@@ -111,12 +111,12 @@ def test_missing_library_fails_loudly(monkeypatch):
         _abi.load()
 
 
-def test_copy_if_rejects_narrow_elements_clearly():
-    # copy_if exists for 4- and 8-byte elements only; the Python mirror says
-    # so before any library lookup (no device work: CPU tensors suffice).
+def test_copy_if_rejects_bf16_clearly():
+    # copy_if exists for the integer types, f32 and f64; the Python mirror
+    # says so before any library lookup (no device work: CPU tensors suffice).
     torch = pytest.importorskip("torch")
-    x = torch.zeros(8, dtype=torch.int8)
+    x = torch.zeros(8, dtype=torch.bfloat16)
     for call in (lambda: acpp.copy_if(None, None, x, x, acpp.PRED_GT, 0),
                  lambda: acpp.copy_if_stencil(None, None, x, torch.zeros(8, dtype=torch.uint8), x)):
-        with pytest.raises(TypeError, match="4- or 8-byte"):
+        with pytest.raises(TypeError, match="integer, f32 or f64"):
             call()

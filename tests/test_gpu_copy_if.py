@@ -10,7 +10,7 @@ from gpu_util import dev, host
 
 pytestmark = pytest.mark.gpu
 
-TYPES = [np.int32, np.uint32, np.int64, np.uint64, np.float32, np.float64]
+TYPES = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64, np.float32, np.float64]
 
 
 def _run(torch, q, scratch, x, pred, operand=0, mask=0, stencil=None, config=None, offset=0):
@@ -64,13 +64,13 @@ def test_stencil_and_misaligned(gpu, q, scratch, dtype):
     torch, _ = gpu
     rng = np.random.default_rng(2)
     n = 3 * 1024 * 1024 + 11
-    x = rng.integers(-1000, 1000, n).astype(dtype)
+    x = rng.integers(-1000, 1000, n).astype(dtype)  # narrow types wrap
     st = (rng.random(n) < 0.3).astype(np.uint8)
     out, c = _run(torch, q, scratch, x, None, stencil=st)
     ref, rc = O.copy_if(x, O.PRED_STENCIL, stencil=st)
     assert c == rc and np.array_equal(out, ref)
     ref_st, rc_st = ref, rc
-    for offset in (1, 3):  # input not 16-byte aligned: scalar loads
+    for offset in (1, 3):  # input not 4-byte aligned: scalar loads
         out, c = _run(torch, q, scratch, x, acpp.PRED_GT, dtype(0), offset=offset)
         ref, rc = O.copy_if(x, O.PRED_GT, dtype(0))
         assert c == rc and np.array_equal(out, ref), offset
