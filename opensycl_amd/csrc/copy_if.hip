@@ -301,7 +301,35 @@ __device__ __forceinline__ void copy_if_tile(const params<T> &p, uint32_t tile, 
     const uint32_t off = sh.wave[seg];
     const uint32_t end = (seg + 1 < R * nw) ? sh.wave[seg + 1] : count;
     const T *src = s_stage + seg * SEG;
-    for (uint32_t j = lane; j < end - off; j += 64) dst[off + j] = src[j];
+    if constexpr (sizeof(T) == 1) {
+      // 4-byte stores of EPW elements: the elements before the segment's
+      // first 4-byte aligned output address and those after its last whole
+      // word are stored one by one (the words around them hold elements of
+      // neighbouring segments); each word's source bytes are two aligned
+      // LDS words shifted together.  (2-byte elements measured slower this
+      // way: one store per element stays.)
+      constexpr uint32_t EPW = 4 / sizeof(T);
+      const uint32_t len = end - off;
+      T *d = dst + off;
+      const uint32_t mis = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(d) / sizeof(T)) % EPW;
+      const uint32_t head = min(len, (EPW - mis) % EPW);
+      if (lane < head) d[lane] = src[lane];
+      const uint32_t words = (len - head) / EPW;
+      uint32_t *dw = reinterpret_cast<uint32_t *>(d + head);
+      const uint32_t *lds32 = reinterpret_cast<const uint32_t *>(s_stage);
+      const uint32_t b0 = static_cast<uint32_t>((seg * SEG + head) * sizeof(T));  // LDS byte of word 0
+      for (uint32_t j = lane; j < words; j += 64) {
+        const uint32_t b = b0 + j * 4;
+        const uint32_t sh = b & 3u;
+        const uint32_t lo = lds32[b >> 2];
+        const uint32_t hi = lds32[(b >> 2) + (sh ? 1u : 0u)];  // in the segment when sh != 0
+        dw[j] = __builtin_amdgcn_alignbyte(hi, lo, sh);
+      }
+      const uint32_t t0 = head + words * EPW;
+      if (lane < len - t0) d[t0 + lane] = src[t0 + lane];
+    } else {
+      for (uint32_t j = lane; j < end - off; j += 64) dst[off + j] = src[j];
+    }
   }
 }
 
