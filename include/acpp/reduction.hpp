@@ -117,13 +117,13 @@ struct reduction_abi {
                                              reinterpret_cast<AT *const *>(outs), ops,              \
                                              reinterpret_cast<const AT *>(ids), init, nullptr);     \
   }
-ACPP_FACADE_RED(signed char, i8, int8_t, false)
-ACPP_FACADE_RED(short, i16, int16_t, false)
+ACPP_FACADE_RED(signed char, i8, int8_t, true)
+ACPP_FACADE_RED(short, i16, int16_t, true)
 ACPP_FACADE_RED(int, i32, int32_t, true)
 ACPP_FACADE_RED(long, i64, int64_t, true)
 ACPP_FACADE_RED(long long, i64, int64_t, true)
-ACPP_FACADE_RED(unsigned char, u8, uint8_t, false)
-ACPP_FACADE_RED(unsigned short, u16, uint16_t, false)
+ACPP_FACADE_RED(unsigned char, u8, uint8_t, true)
+ACPP_FACADE_RED(unsigned short, u16, uint16_t, true)
 ACPP_FACADE_RED(unsigned int, u32, uint32_t, true)
 ACPP_FACADE_RED(unsigned long, u64, uint64_t, true)
 ACPP_FACADE_RED(unsigned long long, u64, uint64_t, true)
@@ -185,7 +185,7 @@ event parallel_for(queue &q, algorithms::util::allocation_group &scratch, std::s
   static_assert(k >= 2 && k <= 4, "acpp::parallel_for: one to four reductions");
   detail::check_reduction_type<T>();
   static_assert(detail::reduction_abi<T>::fused,
-                "acpp::parallel_for: fused reductions take int32, uint32, int64, uint64, float or double");
+                "acpp::parallel_for: fused reductions take 8/16/32/64-bit integers, float or double");
   const T *inputs[k] = {items.x...};
   T *outs[k] = {items.red.var...};
   const acpp_algorithm_op ops[k] = {algorithms::detail::op_code<Ops, T>()...};

@@ -501,6 +501,10 @@ acpp_status acpp_segmented_plan_destroy(acpp_segmented_plan *plan);
 /* `outs` and `ops` are host arrays of `count` (1..4) entries; the pointers */
 /* in them are device pointers; inputs may alias.  One launch.              */
 /* ------------------------------------------------------------------------ */
+acpp_status acpp_reduction_multi_i8(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int8_t *const *inputs, uint64_t n, int8_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_u8(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint8_t *const *inputs, uint64_t n, uint8_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_i16(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int16_t *const *inputs, uint64_t n, int16_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_u16(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint16_t *const *inputs, uint64_t n, uint16_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_i32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int32_t *const *inputs, uint64_t n, int32_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_u32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint32_t *const *inputs, uint64_t n, uint32_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_i64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int64_t *const *inputs, uint64_t n, int64_t *const *outs, const acpp_algorithm_op *ops, int initialize_to_identity, const acpp_launch_config *config);
@@ -512,6 +516,10 @@ acpp_status acpp_reduction_multi_f64(acpp_allocation_group *scratch, acpp_stream
    and, when `identities` is not NULL, a given identity per reduction
    (reduction.hpp:214-236): with the flag *outs[k] = op(identities[k],
    fold), which an empty range leaves as identities[k]. */
+acpp_status acpp_reduction_multi_identity_i8(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int8_t *const *inputs, uint64_t n, int8_t *const *outs, const acpp_algorithm_op *ops, const int8_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_u8(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint8_t *const *inputs, uint64_t n, uint8_t *const *outs, const acpp_algorithm_op *ops, const uint8_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_i16(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int16_t *const *inputs, uint64_t n, int16_t *const *outs, const acpp_algorithm_op *ops, const int16_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
+acpp_status acpp_reduction_multi_identity_u16(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint16_t *const *inputs, uint64_t n, uint16_t *const *outs, const acpp_algorithm_op *ops, const uint16_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_identity_i32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int32_t *const *inputs, uint64_t n, int32_t *const *outs, const acpp_algorithm_op *ops, const int32_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_identity_u32(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const uint32_t *const *inputs, uint64_t n, uint32_t *const *outs, const acpp_algorithm_op *ops, const uint32_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);
 acpp_status acpp_reduction_multi_identity_i64(acpp_allocation_group *scratch, acpp_stream_t stream, uint32_t count, const int64_t *const *inputs, uint64_t n, int64_t *const *outs, const acpp_algorithm_op *ops, const int64_t *identities, const int *initialize_to_identity, const acpp_launch_config *config);

@@ -105,7 +105,7 @@ def _signatures() -> dict[str, tuple[list, object]]:
         for tag in ("i32", "u32", "i64", "u64", "f32", "f64"):
             S[f"acpp_atomic_fetch_{op}_strided_{tag}"] = (
                 [vp, vp, vp, c_uint64, vp, c_uint32, c_int, cfg], c_int)
-    for tag in ("i32", "u32", "i64", "u64", "f32", "f64"):
+    for tag in ("i8", "u8", "i16", "u16", "i32", "u32", "i64", "u64", "f32", "f64"):
         S[f"acpp_reduction_multi_{tag}"] = ([vp, vp, c_uint32, vp, c_uint64, vp, vp, c_int, cfg], c_int)
         S[f"acpp_reduction_multi_identity_{tag}"] = ([vp, vp, c_uint32, vp, c_uint64, vp, vp, vp, vp, cfg],
                                                      c_int)

@@ -279,6 +279,10 @@ extern "C" {
                                   identities, initialize_to_identity, config,                          \
                                   "acpp_reduction_multi_identity_" #tag);                             \
   }
+ACPP_DEFINE_MULTI(i8, int8_t)
+ACPP_DEFINE_MULTI(u8, uint8_t)
+ACPP_DEFINE_MULTI(i16, int16_t)
+ACPP_DEFINE_MULTI(u16, uint16_t)
 ACPP_DEFINE_MULTI(i32, int32_t)
 ACPP_DEFINE_MULTI(u32, uint32_t)
 ACPP_DEFINE_MULTI(i64, int64_t)

@@ -63,6 +63,29 @@ def test_four_reductions_aliasing_and_accumulate(gpu, q, scratch, n):
             assert host(o)[0] == O.reduction(x, s, op, init_id), (n, op, init_id)
 
 
+# 1- and 2-byte elements (16 or 8 per 16-byte load; sums and products wrap),
+# every operator, aligned views and views 1 or 3 elements off (scalar head).
+@pytest.mark.parametrize("dtype", [np.int8, np.uint8, np.int16, np.uint16])
+def test_narrow_types(gpu, q, scratch, dtype):
+    torch, _ = gpu
+    info = np.iinfo(dtype)
+    rng = np.random.default_rng(3)
+    for n in (1, 1000, (1 << 22) + 7):
+        x = rng.integers(info.min, info.max + 1, n + 3).astype(dtype)
+        xd_full = dev(torch, x)
+        for off in (0, 1, 3):
+            xs, xd = x[off:off + n], xd_full[off:off + n]
+            for ops in ([acpp.PLUS, acpp.MIN, acpp.MAX, acpp.BIT_XOR],
+                        [acpp.MULTIPLY, acpp.BIT_AND, acpp.BIT_OR, acpp.LOGICAL_OR]):
+                starts = [dtype(5), dtype(7), dtype(9), dtype(1)]
+                for init_id in (True, False):
+                    outs = [dev(torch, np.array([s])) for s in starts]
+                    acpp.reduction_multi(q, scratch, [xd] * 4, outs, ops, initialize_to_identity=init_id)
+                    q.wait()
+                    for o, op, s in zip(outs, ops, starts):
+                        assert host(o)[0] == O.reduction(xs, s, op, init_id), (dtype, n, off, op, init_id)
+
+
 @pytest.mark.parametrize("offset", [0, 1, 3])
 def test_unaligned_float_inputs(gpu, q, scratch, offset):
     torch, _ = gpu
