@@ -86,7 +86,7 @@ def test_stencil_and_misaligned(gpu, q, scratch, dtype):
         assert c == rc_st and np.array_equal(host(d)[:c], ref_st), soff
 
 
-@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("dtype", [np.int8, np.int16, np.int32, np.int64])
 @pytest.mark.parametrize("kind", ["all", "none", "runs"])
 def test_full_empty_and_run_selections(gpu, q, scratch, dtype, kind):
     # Every element of every wave row kept (each row's LDS segment filled to
@@ -107,19 +107,24 @@ def test_full_empty_and_run_selections(gpu, q, scratch, dtype, kind):
     assert c == rc and np.array_equal(out, ref)
 
 
-@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("dtype", [np.int8, np.uint8, np.int16, np.int32, np.int64])
 def test_unaligned_output(gpu, q, scratch, dtype):
-    # output not 16-byte aligned: element-wise stores
+    # outputs 1, 2 and 3 elements past an aligned address (1-byte elements
+    # are stored four to a 4-byte store between byte-wise segment ends);
+    # the elements around the output stay untouched
     torch, _ = gpu
     x = (np.arange(1_000_003) % 97).astype(dtype)
     xd = dev(torch, x)
-    d_full = torch.empty(x.size + 4, dtype=xd.dtype, device="cuda")
-    cnt64 = torch.zeros(1, dtype=torch.int64, device="cuda")
-    acpp.copy_if(q, scratch, xd, d_full[1:], acpp.PRED_LT, dtype(40), 0, cnt64.view(torch.uint64))
-    q.wait()
     ref, rc = O.copy_if(x, O.PRED_LT, dtype(40))
-    assert int(cnt64.item()) == rc
-    assert np.array_equal(host(d_full)[1:1 + rc], ref)
+    for off in (1, 2, 3):
+        d_full = torch.full((x.size + 8,), 99, dtype=xd.dtype, device="cuda")
+        cnt64 = torch.zeros(1, dtype=torch.int64, device="cuda")
+        acpp.copy_if(q, scratch, xd, d_full[off:], acpp.PRED_LT, dtype(40), 0, cnt64.view(torch.uint64))
+        q.wait()
+        assert int(cnt64.item()) == rc, off
+        got = host(d_full)
+        assert np.array_equal(got[off:off + rc], ref), off
+        assert np.all(got[:off] == 99) and np.all(got[off + rc:] == 99), off
 
 
 @pytest.mark.parametrize("cfg", [(64, 0), (256, 0), (512, 8), (1024, 8)])
