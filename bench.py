@@ -510,6 +510,48 @@ def extras_leg(acpp, q, cache, log2n):
             "gbs": round(8 * n / (avg * 1e-6) / 1e9, 1),
             "matches_torch": bool(torch.equal(di, torch.cumsum(-xi, 0, dtype=torch.int32)))}
         del xi, di, sel, o32
+        # 8- and 16-bit element types: packed-pair scans (plain and with a
+        # transform), 1-byte sums four bytes at a time, copy_if of narrow
+        # elements; checked against torch (sums and scans modulo 2^8)
+        x8 = torch.randint(-128, 128, (n,), dtype=torch.int8, device="cuda", generator=gen)
+        d8 = torch.empty_like(x8)
+        o8 = torch.zeros(1, dtype=torch.int8, device="cuda")
+        csum8 = torch.cumsum(x8.to(torch.int64), 0)
+
+        def to8(t):
+            return (t & 0xff).to(torch.uint8).view(torch.int8)
+        for name, fn, nbytes, check in (
+                ("inclusive_scan_i8_plus", lambda: acpp.inclusive_scan(q, g, x8, d8), 2 * n,
+                 lambda: torch.equal(d8, to8(csum8))),
+                ("transform_inclusive_scan_i8_negate",
+                 lambda: acpp.transform_inclusive_scan(q, g, x8, d8, acpp.PLUS, acpp.NEGATE), 2 * n,
+                 lambda: torch.equal(d8, to8(-csum8))),
+                ("transform_reduce_i8_square",
+                 lambda: acpp.transform_reduce_unary(q, g, x8, o8, 0, acpp.PLUS, acpp.SQUARE), n,
+                 lambda: int(o8.item()) & 0xff == int((x8.to(torch.int64) ** 2).sum().item()) & 0xff)):
+            fn()
+            g.release()
+            avg, _ = timed(q, fn, 10, release=g.release)
+            res[name] = {"avg_us": round(avg, 2), "algorithmic_bytes": nbytes,
+                         "gbs": round(nbytes / (avg * 1e-6) / 1e9, 1), "matches_torch": bool(check())}
+        del csum8, d8, o8
+        for tag, xs in (("i8", x8), ("i16", torch.randint(-(1 << 15), 1 << 15, (n,), dtype=torch.int16,
+                                                        device="cuda", generator=gen))):
+            ds = torch.empty_like(xs)
+
+            def runc8():
+                acpp.copy_if(q, g, xs, ds, acpp.PRED_MASK_EQ, 0, 1, cnt64.view(torch.uint64))
+            runc8()
+            g.release()
+            avg, _ = timed(q, runc8, 10, release=g.release)
+            c = int(cnt64.item())
+            es = xs.element_size()
+            res[f"copy_if_{tag}_even"] = {
+                "avg_us": round(avg, 2), "selected": c, "algorithmic_bytes": es * (n + c),
+                "gbs": round(es * (n + c) / (avg * 1e-6) / 1e9, 1),
+                "output_matches_torch": bool(torch.equal(ds[:c], xs[(xs & 1) == 0]))}
+            del ds
+        del x8, xs
         vals = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32, device="cuda", generator=gen)
         tmax = {}
         for strat_name, strat in (("privatized", acpp.ATOMICS_PRIVATIZED), ("global", acpp.ATOMICS_GLOBAL)):
