@@ -194,17 +194,22 @@ __device__ __forceinline__ void copy_if_tile(const params<T> &p, uint32_t tile, 
     using U = std::make_unsigned_t<T>;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      // each element ORed into its word as it arrives (a select per
-      // element kept all 128 loads of a row set live and spilled)
-      uint32_t d[4] = {0u, 0u, 0u, 0u};
+      const uint32_t first = static_cast<uint32_t>(r * B + tid) * VEC;
+      if (p.vec_io && first + VEC <= in_range) {
+        // a whole vector inside the input: one load
+        q[r] = *reinterpret_cast<const scan_detail::u32x4_a4 *>(in_tile + first);
+      } else {
+        // the vector across the end (or an input not 4-byte aligned):
+        // element by element, each ORed into its word
+        uint32_t d[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-      for (int k = 0; k < VEC; ++k) {
-        const uint32_t i = static_cast<uint32_t>(r * B + tid) * VEC + k;
-        constexpr int sz = static_cast<int>(sizeof(T));
-        if (i < in_range)
-          d[(k * sz) >> 2] |= static_cast<uint32_t>(static_cast<U>(in_tile[i])) << (8 * ((k * sz) & 3));
+        for (int k = 0; k < VEC; ++k) {
+          constexpr int sz = static_cast<int>(sizeof(T));
+          if (first + k < in_range)
+            d[(k * sz) >> 2] |= static_cast<uint32_t>(static_cast<U>(in_tile[first + k])) << (8 * ((k * sz) & 3));
+        }
+        q[r] = u32x4{d[0], d[1], d[2], d[3]};
       }
-      q[r] = u32x4{d[0], d[1], d[2], d[3]};
     }
   } else {
 #pragma unroll
